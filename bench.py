@@ -1,0 +1,246 @@
+"""cholqr2 throughput on one MI355X: rows/s of CholeskyQR2 on a 10,077,696 x 512
+ComplexF64 basis (the 216^3 periodic Helmholtz stencil applied to a seeded
+GhostPanels matrix, panels of 32 columns), resident in HBM, FP64, plus the
+mixed-precision first pass against FP64-only.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one cholqr2 of the whole basis. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "funicular.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "cholqr2! rows/s at N=1e7,k=512; mixed vs FP64 speedup at ‖QᴴQ−I‖≤1e-13"
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+class KernelTimer:
+    """HIP timing events around every Gram and triangular-solve call, recorded
+    on the stream the kernels are launched on."""
+
+    def __init__(self, backend):
+        import ctypes
+        from funicular_hip import _lib
+        from funicular_hip.device_api import current_stream
+        self.backend = backend
+        self.ctypes = ctypes
+        self._lib = _lib
+        self.current_stream = current_stream
+        self.records = {"gram": [], "trsm": []}
+        self.active = False
+        self._orig_gram = backend.gram_accumulate
+        self._orig_trsm = backend.rdiv_upper
+        timer = self
+
+        def gram(C, A, B, alpha, beta, work=None, compute=None):
+            ev = timer._start()
+            r = timer._orig_gram(C, A, B, alpha, beta, work, compute)
+            timer._stop("gram", ev, (A.rows, A.cols, compute))
+            return r
+
+        def trsm(Y, R, work=None, compute=None):
+            ev = timer._start()
+            r = timer._orig_trsm(Y, R, work, compute)
+            timer._stop("trsm", ev, (Y.rows, Y.cols, compute))
+            return r
+        backend.gram_accumulate = gram
+        backend.rdiv_upper = trsm
+
+    def _event(self):
+        e = self.ctypes.c_void_p()
+        self._lib.call("fhip_event_create", self.ctypes.byref(e), 0)
+        return e.value
+
+    def _start(self):
+        if not self.active:
+            return None
+        e = self._event()
+        self._lib.call("fhip_event_record", e, self.current_stream())
+        return e
+
+    def _stop(self, kind, start, shape):
+        if start is None:
+            return
+        e = self._event()
+        self._lib.call("fhip_event_record", e, self.current_stream())
+        self.records[kind].append((start, e, shape))
+
+    def collect(self):
+        """(kind, compute) -> list of (ms, rows, k)."""
+        out = {}
+        for kind, recs in self.records.items():
+            for start, stop, (rows, k, compute) in recs:
+                self._lib.call("fhip_event_sync", stop)
+                ms = self.ctypes.c_float()
+                self._lib.call("fhip_event_elapsed", self.ctypes.byref(ms), start, stop)
+                out.setdefault((kind, compute is not None), []).append((ms.value, rows, k))
+                self._lib.call("fhip_event_destroy", start)
+                self._lib.call("fhip_event_destroy", stop)
+        self.records = {"gram": [], "trsm": []}
+        return out
+
+
+def gram_flops(rows, k):
+    """LAPACK Working Note 41 count for a Hermitian k x k Gram over `rows` rows."""
+    return 4.0 * rows * k * (k + 1)
+
+
+def trsm_flops(rows, k):
+    return 4.0 * rows * k * k + 2.0 * rows * k
+
+
+def summarize(samples, flops_fn):
+    ms = np.array([s[0] for s in samples])
+    flops = np.array([flops_fn(s[1], s[2]) for s in samples])
+    avg_ms = float(ms.mean())
+    tflops = float(flops.sum() / (ms.sum() * 1e-3) / 1e12)
+    return {"launches": len(samples), "avg_ms": avg_ms, "tflops": tflops,
+            "flops_per_launch": float(flops.mean())}
+
+
+def cpu_baseline(k, w, rows, repeats=2):
+    """The oracle's cholqr2 (the reference CPU backend's algorithm: row-block
+    Grams, host Cholesky, row-block triangular solves, FP64 LAPACK/BLAS) timed
+    on this host's cores over a bounded sample of rows of the same workload."""
+    from oracle import ghost as OG
+    from oracle import operators as OO
+    from oracle import panelblas as OB
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max((i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"),
+                      default=os.cpu_count())
+    except Exception:
+        threads = os.cpu_count()
+    n1 = 8
+    n2 = 128
+    n3 = rows // (n1 * n2)
+    N = n1 * n2 * n3
+    Y0 = OB.panelmul(lambda X: OO.helmholtz_apply(X, (n1, n2, n3), OO.helmholtz_diagonal()),
+                     OG.matrix(np.complex128, N, k, 0xDEADBEEF), w)
+    best = None
+    for _ in range(repeats):
+        t = time.perf_counter()
+        OB.cholqr2(Y0, w)
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    return {"value": N / best, "unit": "rows/s", "cores": int(threads), "kind": "port",
+            "sample": f"oracle cholqr2 (numpy/LAPACK FP64, reference step order) of a {N}x{k} ComplexF64 "
+                      f"Helmholtz-applied ghost basis, panels of {w}, best of {repeats}: {best:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=216, help="Helmholtz grid edge; N = grid^3")
+    ap.add_argument("--k", type=int, default=512)
+    ap.add_argument("--w", type=int, default=32)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=1 << 17)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path shards by rows with one all-reduce of the k x k Gram; see the design notes).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import funicular_hip as F
+    from funicular_hip.panelblas import cholqr2_mixed
+
+    backend = F.hip_backend()
+    n = args.grid
+    N, k, w = n ** 3, args.k, args.w
+    log(f"bench: N={N} k={k} w={w} on {backend.arch}")
+    plan = F.ResidencyPlan(backend=backend, device_budget=240 * 1024 ** 3, host_budget=1 << 30,
+                           device_resident=True)
+    d = 6.0 - 0.09 * (1 + 0.05j)
+    G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
+    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+
+    def regenerate():
+        backend.sync_all()
+        t = time.perf_counter()
+        F.panelmul(Y, G, omega)
+        backend.sync_all()
+        return time.perf_counter() - t
+
+    t_panelmul = regenerate()
+    log(f"bench: panelmul {t_panelmul:.3f} s")
+    timer = KernelTimer(backend)
+
+    def run(mode, steps, warmup):
+        fn = F.cholqr2 if mode == "fp64" else cholqr2_mixed
+        regenerate()
+        first = fn(Y)  # from the real basis: accuracy
+        backend.sync_all()
+        orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+        for _ in range(max(warmup - 1, 0)):
+            fn(Y)
+        backend.sync_all()
+        timer.active = True
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn(Y)
+        backend.sync_all()
+        dt = time.perf_counter() - t
+        timer.active = False
+        log(f"bench: {mode} {steps} x cholqr2 in {dt:.3f} s, ||Q^H Q - I||_F = {orth:.2e}")
+        return dt, orth, first, timer.collect()
+
+    dt64, orth64, R64, k64 = run("fp64", args.steps, args.warmup)
+    ms_step = dt64 / args.steps * 1e3
+    value = N * args.steps / dt64
+    gram64 = summarize(k64[("gram", False)], gram_flops)
+    trsm64 = summarize(k64[("trsm", False)], trsm_flops)
+    record = {
+        "metric": METRIC, "value": value, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (complex)", "data": "synthetic",
+        "config": {"workload": f"cholqr2 of a device-resident {N}x{k} ComplexF64 basis: {n}^3 periodic "
+                               f"Helmholtz stencil applied to GhostPanels(seed 0xdeadbeef), panels of {w}",
+                   "N": N, "k": k, "panel_width": w, "row_block": F.row_block_height(N, k, w)},
+        "roofline": {"bound": "mfma", "kernel": "gram_partial_kernel + gram_reduce_kernel (FP64)",
+                     "achieved": gram64["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": gram64["tflops"] / FP64_PEAK_TFLOPS, "traffic": None,
+                     "avg_launch_ms": gram64["avg_ms"], "flops_per_launch": gram64["flops_per_launch"]},
+        "trsm": {"kernel": "trsm_diag_inv_kernel + trsm_runn_kernel (FP64)", "achieved": trsm64["tflops"],
+                 "peak": FP64_PEAK_TFLOPS, "frac": trsm64["tflops"] / FP64_PEAK_TFLOPS,
+                 "avg_launch_ms": trsm64["avg_ms"]},
+        "errors": {"fp64_orthogonality": orth64},
+        "panelmul_s": t_panelmul,
+    }
+    if not args.no_mixed:
+        dtm, orthm, Rm, km = run("mixed", args.steps, args.warmup)
+        record["mixed"] = {"rows_per_s": N * args.steps / dtm, "ms_per_step": dtm / args.steps * 1e3,
+                           "speedup_vs_fp64": dt64 / dtm, "orthogonality": orthm,
+                           "gram_fp32_tflops": summarize(km[("gram", True)], gram_flops)["tflops"],
+                           "trsm_fp32_tflops": summarize(km[("trsm", True)], trsm_flops)["tflops"],
+                           "R_rel_diff_vs_fp64": float(np.linalg.norm(Rm - R64) / np.linalg.norm(R64))}
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline")
+        record["cpu_baseline"] = cpu_baseline(k, w, args.cpu_rows)
+    print(json.dumps(record), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,247 @@
+// Elementwise device kernels: eltype conversion (convert_copy!,
+// src/kernels.jl:7-20, used by convert_staged! src/panels.jl:365-371 and
+// rightmul! src/panelblas.jl:216), a fill that stands in for memset, and the
+// two matrix-free operators of the benchmark configurations, applied to a
+// whole panel (the panel_capable path of apply_operator!, src/operator.jl:80).
+#include "fhip_internal.hpp"
+
+namespace fhip {
+
+template <typename TD, bool CD, typename TS, bool CS>
+__global__ __launch_bounds__(256) void convert_kernel(TD *__restrict__ dst, int64_t ldd,
+                                                      const TS *__restrict__ src, int64_t lds, int64_t rows,
+                                                      int64_t cols) {
+  const int64_t n = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % rows;
+    const int64_t j = e / rows;
+    TS re, im;
+    load_elem<TS, TS, CS>(src, i + j * lds, true, re, im);
+    store_elem<TD, CD>(dst, i + j * ldd, (TD)re, (TD)im);
+  }
+}
+
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void fill_kernel(T *__restrict__ dst, int64_t ld, int64_t rows, int64_t cols,
+                                                   T vr, T vi) {
+  const int64_t n = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % rows;
+    const int64_t j = e / rows;
+    store_elem<T, C>(dst, i + j * ld, vr, vi);
+  }
+}
+
+static unsigned grid_for(int64_t n) {
+  int64_t g = cdiv64(n, 256);
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+template <typename TD, bool CD>
+static int convert_from(void *dst, int64_t ldd, const void *src, int src_type, int64_t lds, int64_t rows,
+                        int64_t cols, hipStream_t s) {
+  const unsigned g = grid_for(rows * cols);
+  switch (src_type) {
+    case FHIP_F32:
+      hipLaunchKernelGGL((convert_kernel<TD, CD, float, false>), dim3(g), dim3(256), 0, s, (TD *)dst, ldd,
+                         (const float *)src, lds, rows, cols);
+      break;
+    case FHIP_F64:
+      hipLaunchKernelGGL((convert_kernel<TD, CD, double, false>), dim3(g), dim3(256), 0, s, (TD *)dst, ldd,
+                         (const double *)src, lds, rows, cols);
+      break;
+    case FHIP_C32:
+      hipLaunchKernelGGL((convert_kernel<TD, CD, float, true>), dim3(g), dim3(256), 0, s, (TD *)dst, ldd,
+                         (const float *)src, lds, rows, cols);
+      break;
+    case FHIP_C64:
+      hipLaunchKernelGGL((convert_kernel<TD, CD, double, true>), dim3(g), dim3(256), 0, s, (TD *)dst, ldd,
+                         (const double *)src, lds, rows, cols);
+      break;
+  }
+  return check_launch("convert_kernel");
+}
+
+// Periodic tridiagonal stencil, one thread per (row, column) of the panel.
+// (Gx)_i = d_i x_i + o (x_{i-1} + x_{i+1}); the adjoint conjugates d and o.
+template <typename T>
+__global__ __launch_bounds__(256) void tridiag_kernel(T *__restrict__ Y, int64_t ldy, const T *__restrict__ X,
+                                                      int64_t ldx, int64_t N, int64_t w, T dr, T di, T orr,
+                                                      T oi, const double *__restrict__ diag) {
+  const int64_t n = N * w;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % N;
+    const int64_t j = e / N;
+    const int64_t im1 = i == 0 ? N - 1 : i - 1;
+    const int64_t ip1 = i == N - 1 ? 0 : i + 1;
+    T xr, xi, ar, ai, br, bi;
+    load_elem<T, T, true>(X, i + j * ldx, true, xr, xi);
+    load_elem<T, T, true>(X, im1 + j * ldx, true, ar, ai);
+    load_elem<T, T, true>(X, ip1 + j * ldx, true, br, bi);
+    T cr = dr, ci = di;
+    if (diag) cr += (T)diag[i];
+    const T nr = ar + br, ni = ai + bi;
+    const T yr = cr * xr - ci * xi + orr * nr - oi * ni;
+    const T yi = cr * xi + ci * xr + orr * ni + oi * nr;
+    store_elem<T, true>(Y, i + j * ldy, yr, yi);
+  }
+}
+
+// Periodic 7-point stencil on n1 x n2 x n3, linear index i + n1 (j + n2 l).
+// A workgroup sweeps a run of consecutive i, so the +-1 neighbours share
+// cache lines and the +-n1, +-n1 n2 neighbours are reused from L2.
+template <typename T>
+__global__ __launch_bounds__(256) void helmholtz_kernel(T *__restrict__ Y, int64_t ldy,
+                                                        const T *__restrict__ X, int64_t ldx, int64_t n1,
+                                                        int64_t n2, int64_t n3, int64_t w, T dr, T di, T orr,
+                                                        T oi) {
+  const int64_t N = n1 * n2 * n3;
+  const int64_t n = N * w;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t p = e % N;
+    const int64_t col = e / N;
+    const int64_t i = p % n1;
+    const int64_t jl = p / n1;
+    const int64_t j = jl % n2;
+    const int64_t l = jl / n2;
+    const int64_t base = col * ldx;
+    const int64_t s1 = 1, s2 = n1, s3 = n1 * n2;
+    const int64_t im = (i == 0 ? p + (n1 - 1) * s1 : p - s1);
+    const int64_t ip = (i == n1 - 1 ? p - (n1 - 1) * s1 : p + s1);
+    const int64_t jm = (j == 0 ? p + (n2 - 1) * s2 : p - s2);
+    const int64_t jp = (j == n2 - 1 ? p - (n2 - 1) * s2 : p + s2);
+    const int64_t lm = (l == 0 ? p + (n3 - 1) * s3 : p - s3);
+    const int64_t lp = (l == n3 - 1 ? p - (n3 - 1) * s3 : p + s3);
+    T xr, xi, sr = 0, si = 0, tr, ti;
+    load_elem<T, T, true>(X, base + p, true, xr, xi);
+    load_elem<T, T, true>(X, base + im, true, tr, ti);
+    sr += tr;
+    si += ti;
+    load_elem<T, T, true>(X, base + ip, true, tr, ti);
+    sr += tr;
+    si += ti;
+    load_elem<T, T, true>(X, base + jm, true, tr, ti);
+    sr += tr;
+    si += ti;
+    load_elem<T, T, true>(X, base + jp, true, tr, ti);
+    sr += tr;
+    si += ti;
+    load_elem<T, T, true>(X, base + lm, true, tr, ti);
+    sr += tr;
+    si += ti;
+    load_elem<T, T, true>(X, base + lp, true, tr, ti);
+    sr += tr;
+    si += ti;
+    const T yr = dr * xr - di * xi + orr * sr - oi * si;
+    const T yi = dr * xi + di * xr + orr * si + oi * sr;
+    store_elem<T, true>(Y, p + col * ldy, yr, yi);
+  }
+}
+
+}  // namespace fhip
+
+using namespace fhip;
+
+extern "C" int fhip_convert_copy(void *dst, int dst_type, int64_t ldd, const void *src, int src_type,
+                                 int64_t lds, int64_t rows, int64_t cols, void *stream) {
+  if (!valid_type(dst_type) || !valid_type(src_type))
+    return fail(FHIP_E_UNSUPPORTED, "fhip_convert_copy: unknown eltype");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_convert_copy: negative dimension");
+  if (rows == 0 || cols == 0) return FHIP_OK;
+  if (!dst || !src) return fail(FHIP_E_ARGUMENT, "fhip_convert_copy: null pointer");
+  if (ldd < rows || lds < rows) return fail(FHIP_E_ARGUMENT, "fhip_convert_copy: leading dimension smaller than the rows");
+  if (is_complex(src_type) && !is_complex(dst_type))
+    return fail(FHIP_E_ARGUMENT, "fhip_convert_copy: a complex source cannot be converted to a real destination");
+  hipStream_t s = (hipStream_t)stream;
+  switch (dst_type) {
+    case FHIP_F32: return convert_from<float, false>(dst, ldd, src, src_type, lds, rows, cols, s);
+    case FHIP_F64: return convert_from<double, false>(dst, ldd, src, src_type, lds, rows, cols, s);
+    case FHIP_C32: return convert_from<float, true>(dst, ldd, src, src_type, lds, rows, cols, s);
+    case FHIP_C64: return convert_from<double, true>(dst, ldd, src, src_type, lds, rows, cols, s);
+  }
+  return fail(FHIP_E_UNSUPPORTED, "fhip_convert_copy: unknown eltype");
+}
+
+extern "C" int fhip_fill(void *dst, int type, int64_t ld, int64_t rows, int64_t cols, double value_re,
+                         double value_im, void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_fill: unknown eltype");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_fill: negative dimension");
+  if (rows == 0 || cols == 0) return FHIP_OK;
+  if (!dst) return fail(FHIP_E_ARGUMENT, "fhip_fill: null pointer");
+  if (ld < rows) return fail(FHIP_E_ARGUMENT, "fhip_fill: leading dimension smaller than the rows");
+  const unsigned g = grid_for(rows * cols);
+  hipStream_t s = (hipStream_t)stream;
+  switch (type) {
+    case FHIP_F32:
+      hipLaunchKernelGGL((fill_kernel<float, false>), dim3(g), dim3(256), 0, s, (float *)dst, ld, rows, cols,
+                         (float)value_re, 0.0f);
+      break;
+    case FHIP_F64:
+      hipLaunchKernelGGL((fill_kernel<double, false>), dim3(g), dim3(256), 0, s, (double *)dst, ld, rows, cols,
+                         value_re, 0.0);
+      break;
+    case FHIP_C32:
+      hipLaunchKernelGGL((fill_kernel<float, true>), dim3(g), dim3(256), 0, s, (float *)dst, ld, rows, cols,
+                         (float)value_re, (float)value_im);
+      break;
+    case FHIP_C64:
+      hipLaunchKernelGGL((fill_kernel<double, true>), dim3(g), dim3(256), 0, s, (double *)dst, ld, rows, cols,
+                         value_re, value_im);
+      break;
+  }
+  return check_launch("fill_kernel");
+}
+
+extern "C" int fhip_stencil_tridiag(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t N, int64_t w,
+                                    double d_re, double d_im, double o_re, double o_im, const double *diag,
+                                    int adjoint, int type, void *stream) {
+  if (type != FHIP_C64 && type != FHIP_C32)
+    return fail(FHIP_E_UNSUPPORTED, "fhip_stencil_tridiag: the stencil operators are complex");
+  if (N < 0 || w < 0) return fail(FHIP_E_ARGUMENT, "fhip_stencil_tridiag: negative dimension");
+  if (N == 0 || w == 0) return FHIP_OK;
+  if (!Y || !X) return fail(FHIP_E_ARGUMENT, "fhip_stencil_tridiag: null pointer");
+  if (X == Y) return fail(FHIP_E_ARGUMENT, "fhip_stencil_tridiag: output aliases input");
+  if (ldx < N || ldy < N) return fail(FHIP_E_ARGUMENT, "fhip_stencil_tridiag: leading dimension smaller than N");
+  if (adjoint) {
+    d_im = -d_im;
+    o_im = -o_im;
+  }
+  const unsigned g = grid_for(N * w);
+  hipStream_t s = (hipStream_t)stream;
+  if (type == FHIP_C64)
+    hipLaunchKernelGGL((tridiag_kernel<double>), dim3(g), dim3(256), 0, s, (double *)Y, ldy, (const double *)X,
+                       ldx, N, w, d_re, d_im, o_re, o_im, diag);
+  else
+    hipLaunchKernelGGL((tridiag_kernel<float>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X, ldx,
+                       N, w, (float)d_re, (float)d_im, (float)o_re, (float)o_im, diag);
+  return check_launch("tridiag_kernel");
+}
+
+extern "C" int fhip_stencil_helmholtz3d(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t n1,
+                                        int64_t n2, int64_t n3, int64_t w, double d_re, double d_im,
+                                        double o_re, double o_im, int adjoint, int type, void *stream) {
+  if (type != FHIP_C64 && type != FHIP_C32)
+    return fail(FHIP_E_UNSUPPORTED, "fhip_stencil_helmholtz3d: the stencil operators are complex");
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0 || w < 0)
+    return fail(FHIP_E_ARGUMENT, "fhip_stencil_helmholtz3d: grid dimensions must be positive");
+  if (w == 0) return FHIP_OK;
+  const int64_t N = n1 * n2 * n3;
+  if (!Y || !X) return fail(FHIP_E_ARGUMENT, "fhip_stencil_helmholtz3d: null pointer");
+  if (X == Y) return fail(FHIP_E_ARGUMENT, "fhip_stencil_helmholtz3d: output aliases input");
+  if (ldx < N || ldy < N) return fail(FHIP_E_ARGUMENT, "fhip_stencil_helmholtz3d: leading dimension smaller than N");
+  if (adjoint) {
+    d_im = -d_im;
+    o_im = -o_im;
+  }
+  const unsigned g = grid_for(N * w);
+  hipStream_t s = (hipStream_t)stream;
+  if (type == FHIP_C64)
+    hipLaunchKernelGGL((helmholtz_kernel<double>), dim3(g), dim3(256), 0, s, (double *)Y, ldy,
+                       (const double *)X, ldx, n1, n2, n3, w, d_re, d_im, o_re, o_im);
+  else
+    hipLaunchKernelGGL((helmholtz_kernel<float>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X,
+                       ldx, n1, n2, n3, w, (float)d_re, (float)d_im, (float)o_re, (float)o_im);
+  return check_launch("helmholtz_kernel");
+}

@@ -1,0 +1,165 @@
+// Shared internals of libfunicular_hip.so: error helpers, element traits and
+// the gfx950 matrix-core wrappers the kernels are built from.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/funicular_hip.h"
+
+namespace fhip {
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t err, const char *what);
+int check_launch(const char *what);
+size_t eltype_size(int type);
+
+inline bool is_complex(int type) { return type == FHIP_C32 || type == FHIP_C64; }
+inline bool is_double(int type) { return type == FHIP_F64 || type == FHIP_C64; }
+inline bool valid_type(int type) { return type >= FHIP_F32 && type <= FHIP_C64; }
+
+inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core product per call, D = A*B + C, on a 64-wide wave.
+// Operand layout: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
+// the result holds D[4 * (l / 16) + r][l % 16] in element r.
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef v4d acc;
+  __device__ static inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static inline acc mma(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  typedef v4f acc;
+  __device__ static inline acc zero() { return acc{0.0f, 0.0f, 0.0f, 0.0f}; }
+  __device__ static inline acc mma(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// How the complex product of one K step combines its four real products.
+enum ProductMode {
+  PROD_CONJ_A = 0,  // acc += conj(a) * b   (Gram: A^H B)
+  PROD_ADD = 1,     // acc += a * b         (D = A C)
+  PROD_SUB = 2      // acc -= a * b         (trailing update of a solve)
+};
+
+// K steps s = s_begin, s_begin + s_stride, ... < s_end of a wave's MB x NB
+// block tile. As/Bs are LDS planes laid out [k][m] and [k][n] with row strides
+// lda_s/ldb_s; plane 1 holds imaginary parts when CPLX. The wave's tile starts
+// at column mi0 of As and column nj0 of Bs.
+template <typename T, bool CPLX, int MB, int NB, int MODE>
+__device__ inline void mma_steps(const T *__restrict__ As_re, const T *__restrict__ As_im, int lda_s,
+                                 int mi0, const T *__restrict__ Bs_re, const T *__restrict__ Bs_im,
+                                 int ldb_s, int nj0, int s_begin, int s_end, int s_stride,
+                                 typename Mfma<T>::acc (&acc_re)[MB][NB],
+                                 typename Mfma<T>::acc (&acc_im)[MB][NB]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15;
+  const int lk = lane >> 4;
+  for (int s = s_begin; s < s_end; s += s_stride) {
+    const int kr = 4 * s + lk;
+    T ar[MB], ai[MB], br[NB], bi[NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      ar[mb] = As_re[kr * lda_s + mi0 + 16 * mb + li];
+      if (CPLX) ai[mb] = As_im[kr * lda_s + mi0 + 16 * mb + li];
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      br[nb] = Bs_re[kr * ldb_s + nj0 + 16 * nb + li];
+      if (CPLX) bi[nb] = Bs_im[kr * ldb_s + nj0 + 16 * nb + li];
+    }
+    if (!CPLX) {
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const T a = (MODE == PROD_SUB) ? -ar[mb] : ar[mb];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc_re[mb][nb] = Mfma<T>::mma(a, br[nb], acc_re[mb][nb]);
+      }
+    } else {
+      // First half: the products with the real part of a.
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const T a = (MODE == PROD_SUB) ? -ar[mb] : ar[mb];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          acc_re[mb][nb] = Mfma<T>::mma(a, br[nb], acc_re[mb][nb]);
+          acc_im[mb][nb] = Mfma<T>::mma(a, bi[nb], acc_im[mb][nb]);
+        }
+      }
+      // Second half: the products with the imaginary part of a.
+      //   conj(a) b: re += ai bi, im -= ai br
+      //   a b:       re -= ai bi, im += ai br
+      //   -a b:      re += ai bi, im -= ai br
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        const T p = (MODE == PROD_ADD) ? -ai[mb] : ai[mb];
+        const T q = (MODE == PROD_ADD) ? ai[mb] : -ai[mb];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          acc_re[mb][nb] = Mfma<T>::mma(p, bi[nb], acc_re[mb][nb]);
+          acc_im[mb][nb] = Mfma<T>::mma(q, br[nb], acc_im[mb][nb]);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+struct Pair;
+template <>
+struct Pair<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+
+// Loads one element of a real or complex matrix as (re, im) in type T. A
+// complex element is read as one aligned pair (ComplexF64 and ComplexF32 are
+// aligned to their full size).
+template <typename TIn, typename T, bool CPLX>
+__device__ inline void load_elem(const TIn *__restrict__ p, int64_t idx, bool ok, T &re, T &im) {
+  if (!ok) {
+    re = T(0);
+    im = T(0);
+    return;
+  }
+  if (CPLX) {
+    typedef typename Pair<TIn>::type P2;
+    const P2 v = reinterpret_cast<const P2 *>(p)[idx];
+    re = T(v.x);
+    im = T(v.y);
+  } else {
+    re = T(p[idx]);
+    im = T(0);
+  }
+}
+
+// Stores one element of a real or complex matrix.
+template <typename TOut, bool CPLX>
+__device__ inline void store_elem(TOut *__restrict__ p, int64_t idx, TOut re, TOut im) {
+  if (CPLX) {
+    typedef typename Pair<TOut>::type P2;
+    P2 v;
+    v.x = re;
+    v.y = im;
+    reinterpret_cast<P2 *>(p)[idx] = v;
+  } else {
+    p[idx] = re;
+  }
+}
+
+}  // namespace fhip

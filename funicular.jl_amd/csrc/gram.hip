@@ -1,0 +1,341 @@
+// Kernel (a): the tall-skinny Gram C <- alpha A^H B + beta C over m rows,
+// replacing gram_accumulate! (src/kernels.jl:40, semantics :22-31) as called
+// from gram (src/panelblas.jl:33-53) and project (src/panelblas.jl:294).
+//
+// Design (gfx950):
+//  * The output is cut into TILE x TILE tiles (upper-triangle tiles only when
+//    A and B are the same matrix) and the m rows into chunks. One workgroup
+//    owns one (tile, chunk) pair and sums its rows with v_mfma_f64_16x16x4
+//    (or the f32 form for the mixed pass) on the real and imaginary planes,
+//    staging KC-row slices of both operands through LDS.
+//  * Workgroup ids are dealt round-robin to the 8 XCDs, so the id is split as
+//    (xcd, local) and every tile of one chunk runs on the same XCD: the rows
+//    of a chunk come from HBM once and the other tiles hit that XCD's L2.
+//  * Each workgroup writes its tile to a private partial slot; a second kernel
+//    adds the slots in slot order and applies alpha and beta. No atomics, so
+//    the bits depend on (m, ka, kb) only.
+#include "fhip_internal.hpp"
+
+namespace fhip {
+
+template <int TILE>
+struct GramShape;
+template <>
+struct GramShape<64> {
+  static constexpr int KC = 16;     // rows staged per LDS pass
+  static constexpr int KSPLIT = 1;  // waves that split the K steps of one tile
+};
+template <>
+struct GramShape<32> {
+  static constexpr int KC = 32;
+  static constexpr int KSPLIT = 4;
+};
+
+constexpr int GRAM_THREADS = 256;
+constexpr int GRAM_TARGET_WGS = 4096;
+constexpr int GRAM_MIN_CHUNK_ROWS = 256;
+
+// Upper-triangle tile t -> (ti, tj), tiles numbered column by column.
+__host__ __device__ inline void tri_tile(int t, int &ti, int &tj) {
+  int j = 0;
+  while ((j + 1) * (j + 2) / 2 <= t) ++j;
+  tj = j;
+  ti = t - j * (j + 1) / 2;
+}
+
+struct GramPlan {
+  int tile;
+  int ntm, ntn, ntiles;
+  int q;              // chunks = 8 * q
+  int64_t chunk_rows; // multiple of KC
+  int ksplit;
+  int nslots;         // 8 * q * ksplit
+};
+
+static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm) {
+  GramPlan p;
+  p.tile = (ka > 32 || kb > 32) ? 64 : 32;
+  const int kc = p.tile == 64 ? GramShape<64>::KC : GramShape<32>::KC;
+  p.ksplit = p.tile == 64 ? GramShape<64>::KSPLIT : GramShape<32>::KSPLIT;
+  p.ntm = (int)cdiv64(ka, p.tile);
+  p.ntn = (int)cdiv64(kb, p.tile);
+  p.ntiles = herm ? p.ntm * (p.ntm + 1) / 2 : p.ntm * p.ntn;
+  int64_t want = cdiv64(GRAM_TARGET_WGS, p.ntiles);
+  int64_t by_rows = cdiv64(m > 0 ? m : 1, GRAM_MIN_CHUNK_ROWS);
+  int64_t chunks = want < by_rows ? want : by_rows;
+  if (chunks < 1) chunks = 1;
+  p.q = (int)cdiv64(chunks, 8);
+  int64_t rows = cdiv64(m > 0 ? m : 1, 8 * (int64_t)p.q);
+  p.chunk_rows = cdiv64(rows, kc) * kc;
+  p.nslots = 8 * p.q * p.ksplit;
+  return p;
+}
+
+template <typename TIn, typename TC, bool CPLX, int TILE>
+__global__ __launch_bounds__(GRAM_THREADS) void gram_partial_kernel(
+    const TIn *__restrict__ A, int64_t lda, const TIn *__restrict__ B, int64_t ldb, int64_t m, int ka,
+    int kb, int64_t chunk_rows, int ntiles, int ntm, int herm, TC *__restrict__ partial) {
+  constexpr int KC = GramShape<TILE>::KC;
+  constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
+  constexpr int LDS_W = TILE + 4;
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int PER_THREAD = KC * TILE / GRAM_THREADS;  // 4 in both shapes
+  __shared__ TC As[NP][KC * LDS_W];
+  __shared__ TC Bs[NP][KC * LDS_W];
+
+  const int id = blockIdx.x;
+  const int xcd = id & 7;
+  const int local = id >> 3;
+  const int tile = local % ntiles;
+  const int chunk = (local / ntiles) * 8 + xcd;
+  int ti, tj;
+  if (herm) {
+    tri_tile(tile, ti, tj);
+  } else {
+    ti = tile % ntm;
+    tj = tile / ntm;
+  }
+  const int i0 = ti * TILE;
+  const int j0 = tj * TILE;
+  const bool same = herm && ti == tj;
+  const int64_t r0 = (int64_t)chunk * chunk_rows;
+  const int64_t r1 = (r0 + chunk_rows < m) ? r0 + chunk_rows : m;
+
+  const int t = threadIdx.x;
+  const int w = t >> 6;
+  const int lane = t & 63;
+  const int li = lane & 15;
+  const int lk = lane >> 4;
+  // The wave's 32x32 sub-tile and the K steps it owns.
+  const int mi0 = (TILE == 64) ? 32 * (w & 1) : 0;
+  const int nj0 = (TILE == 64) ? 32 * (w >> 1) : 0;
+  const int s_begin = (KSPLIT > 1) ? w : 0;
+  const int s_stride = KSPLIT;
+
+  typedef typename Mfma<TC>::acc acc_t;
+  acc_t acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      acc_re[a][b] = Mfma<TC>::zero();
+      acc_im[a][b] = Mfma<TC>::zero();
+    }
+
+  // Register staging of the next slice while the current one is multiplied.
+  TC pa_re[PER_THREAD], pa_im[PER_THREAD], pb_re[PER_THREAD], pb_im[PER_THREAD];
+  auto fetch = [&](int64_t rb) {
+#pragma unroll
+    for (int e = 0; e < PER_THREAD; ++e) {
+      const int idx = t + GRAM_THREADS * e;
+      const int kk = idx % KC;
+      const int c = idx / KC;
+      const int64_t row = rb + kk;
+      const bool rok = row < r1;
+      load_elem<TIn, TC, CPLX>(A, row + (int64_t)(i0 + c) * lda, rok && (i0 + c) < ka, pa_re[e], pa_im[e]);
+      if (!same)
+        load_elem<TIn, TC, CPLX>(B, row + (int64_t)(j0 + c) * ldb, rok && (j0 + c) < kb, pb_re[e],
+                                 pb_im[e]);
+    }
+  };
+
+  if (r0 < r1) fetch(r0);
+  for (int64_t rb = r0; rb < r1; rb += KC) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER_THREAD; ++e) {
+      const int idx = t + GRAM_THREADS * e;
+      const int kk = idx % KC;
+      const int c = idx / KC;
+      As[0][kk * LDS_W + c] = pa_re[e];
+      if (CPLX) As[NP - 1][kk * LDS_W + c] = pa_im[e];
+      if (!same) {
+        Bs[0][kk * LDS_W + c] = pb_re[e];
+        if (CPLX) Bs[NP - 1][kk * LDS_W + c] = pb_im[e];
+      }
+    }
+    __syncthreads();
+    if (rb + KC < r1) fetch(rb + KC);
+    const TC *bre = same ? As[0] : Bs[0];
+    const TC *bim = same ? As[NP - 1] : Bs[NP - 1];
+    mma_steps<TC, CPLX, 2, 2, PROD_CONJ_A>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
+                                           KC / 4, s_stride, acc_re, acc_im);
+  }
+
+  // Partial slot of this workgroup (and wave, when waves split K).
+  const int slot = chunk * KSPLIT + ((KSPLIT > 1) ? w : 0);
+  TC *out = partial + ((int64_t)slot * ntiles + tile) * (TILE * TILE) * NP;
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int il = mi0 + 16 * mb + 4 * lk + r;
+        const int jl = nj0 + 16 * nb + li;
+        const int64_t off = ((int64_t)jl * TILE + il) * NP;
+        out[off] = acc_re[mb][nb][r];
+        if (CPLX) out[off + 1] = acc_im[mb][nb][r];
+      }
+}
+
+// Adds the partial slots in slot order and applies alpha and beta.
+template <typename TC, typename TOut, bool CPLX, int TILE>
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const TC *__restrict__ partial, int nslots,
+                                                          int ntiles, int ntm, int herm,
+                                                          TOut *__restrict__ C, int64_t ldc, int ka,
+                                                          int kb, double al_re, double al_im,
+                                                          double be_re, double be_im, int beta_zero) {
+  constexpr int NP = CPLX ? 2 : 1;
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)ntiles * TILE * TILE) return;
+  const int tile = (int)(e / (TILE * TILE));
+  const int within = (int)(e % (TILE * TILE));
+  const int jl = within / TILE;
+  const int il = within % TILE;
+  int ti, tj;
+  if (herm) {
+    tri_tile(tile, ti, tj);
+  } else {
+    ti = tile % ntm;
+    tj = tile / ntm;
+  }
+  const int i = ti * TILE + il;
+  const int j = tj * TILE + jl;
+  if (i >= ka || j >= kb) return;
+  // Below the diagonal of a diagonal tile, take the mirrored upper element so
+  // the result is exactly Hermitian.
+  const bool lower = herm && ti == tj && il > jl;
+  const int src = lower ? (il * TILE + jl) : within;
+  double sr = 0.0, si = 0.0;
+  const int64_t stride = (int64_t)ntiles * TILE * TILE * NP;
+  const TC *p = partial + ((int64_t)tile * TILE * TILE + src) * NP;
+  for (int s = 0; s < nslots; ++s) {
+    sr += (double)p[0];
+    if (CPLX) si += (double)p[1];
+    p += stride;
+  }
+  if (lower) si = -si;
+  // The diagonal of A^H A is real.
+  if (herm && i == j) si = 0.0;
+
+  auto store = [&](int64_t idx, double vr, double vi) {
+    double outr = al_re * vr - al_im * vi;
+    double outi = al_re * vi + al_im * vr;
+    if (!beta_zero) {
+      const double cr = (double)C[idx * NP];
+      const double ci = CPLX ? (double)C[idx * NP + 1] : 0.0;
+      outr += be_re * cr - be_im * ci;
+      outi += be_re * ci + be_im * cr;
+    }
+    C[idx * NP] = (TOut)outr;
+    if (CPLX) C[idx * NP + 1] = (TOut)outi;
+  };
+  store((int64_t)i + (int64_t)j * ldc, sr, si);
+  if (herm && ti < tj && j < ka && i < kb) store((int64_t)j + (int64_t)i * ldc, sr, -si);
+}
+
+template <typename TIn, typename TC, typename TOut, bool CPLX, int TILE>
+static int launch_gram(const GramPlan &p, TOut *C, int64_t ldc, const TIn *A, int64_t lda, const TIn *B,
+                       int64_t ldb, int64_t m, int ka, int kb, bool herm, double ar, double ai,
+                       double br, double bi, void *work, hipStream_t stream) {
+  TC *partial = (TC *)work;
+  const int grid = 8 * p.q * p.ntiles;
+  hipLaunchKernelGGL((gram_partial_kernel<TIn, TC, CPLX, TILE>), dim3(grid), dim3(GRAM_THREADS), 0,
+                     stream, A, lda, B, ldb, m, ka, kb, p.chunk_rows, p.ntiles, p.ntm, herm ? 1 : 0,
+                     partial);
+  int st = check_launch("gram_partial_kernel");
+  if (st) return st;
+  const int64_t elems = (int64_t)p.ntiles * TILE * TILE;
+  const int rgrid = (int)cdiv64(elems, 256);
+  const int beta_zero = (br == 0.0 && bi == 0.0) ? 1 : 0;
+  hipLaunchKernelGGL((gram_reduce_kernel<TC, TOut, CPLX, TILE>), dim3(rgrid), dim3(256), 0, stream,
+                     partial, p.nslots, p.ntiles, p.ntm, herm ? 1 : 0, C, ldc, ka, kb, ar, ai, br, bi,
+                     beta_zero);
+  return check_launch("gram_reduce_kernel");
+}
+
+template <typename TIn, typename TC, typename TOut, bool CPLX>
+static int dispatch_gram(const GramPlan &p, void *C, int64_t ldc, const void *A, int64_t lda,
+                         const void *B, int64_t ldb, int64_t m, int ka, int kb, bool herm, double ar,
+                         double ai, double br, double bi, void *work, hipStream_t stream) {
+  if (p.tile == 64)
+    return launch_gram<TIn, TC, TOut, CPLX, 64>(p, (TOut *)C, ldc, (const TIn *)A, lda, (const TIn *)B,
+                                                ldb, m, ka, kb, herm, ar, ai, br, bi, work, stream);
+  return launch_gram<TIn, TC, TOut, CPLX, 32>(p, (TOut *)C, ldc, (const TIn *)A, lda, (const TIn *)B, ldb,
+                                              m, ka, kb, herm, ar, ai, br, bi, work, stream);
+}
+
+static size_t gram_work_bytes(const GramPlan &p, int compute) {
+  const size_t np = is_complex(compute) ? 2 : 1;
+  const size_t real = is_double(compute) ? 8 : 4;
+  return (size_t)p.nslots * p.ntiles * p.tile * p.tile * np * real;
+}
+
+static bool gram_types_ok(int type, int compute) {
+  if (!valid_type(type) || !valid_type(compute)) return false;
+  if (type == compute) return true;
+  return type == FHIP_C64 && compute == FHIP_C32;
+}
+
+}  // namespace fhip
+
+using namespace fhip;
+
+extern "C" int fhip_gram_workspace_bytes(size_t *nbytes, int64_t m, int64_t ka, int64_t kb, int type,
+                                         int compute, int hermitian) {
+  if (!nbytes) return fail(FHIP_E_ARGUMENT, "fhip_gram_workspace_bytes: null output pointer");
+  if (!gram_types_ok(type, compute))
+    return fail(FHIP_E_UNSUPPORTED, "fhip_gram: no kernel for storage type %d with compute type %d", type,
+                compute);
+  if (m < 0 || ka < 0 || kb < 0) return fail(FHIP_E_ARGUMENT, "fhip_gram: negative dimension");
+  if (ka == 0 || kb == 0) {
+    *nbytes = 0;
+    return FHIP_OK;
+  }
+  GramPlan p = gram_plan(m, ka, kb, hermitian != 0 && ka == kb);
+  *nbytes = gram_work_bytes(p, compute);
+  return FHIP_OK;
+}
+
+extern "C" int fhip_gram(void *C, int64_t ldc, const void *A, int64_t lda, const void *B, int64_t ldb,
+                         int64_t m, int64_t ka, int64_t kb, double alpha_re, double alpha_im,
+                         double beta_re, double beta_im, int type, int compute, void *work,
+                         size_t work_bytes, void *stream) {
+  if (!gram_types_ok(type, compute))
+    return fail(FHIP_E_UNSUPPORTED, "fhip_gram: no kernel for storage type %d with compute type %d", type,
+                compute);
+  if (m < 0 || ka < 0 || kb < 0) return fail(FHIP_E_ARGUMENT, "fhip_gram: negative dimension");
+  if (ka == 0 || kb == 0) return FHIP_OK;
+  if (ka > (1 << 20) || kb > (1 << 20)) return fail(FHIP_E_ARGUMENT, "fhip_gram: too many columns");
+  if (!C || !A || !B) return fail(FHIP_E_ARGUMENT, "fhip_gram: null matrix pointer");
+  if (ldc < ka || (m > 0 && (lda < m || ldb < m)))
+    return fail(FHIP_E_ARGUMENT, "fhip_gram: leading dimension smaller than the matrix (ldc=%lld lda=%lld ldb=%lld)",
+                (long long)ldc, (long long)lda, (long long)ldb);
+  const bool herm = (A == B) && (lda == ldb) && (ka == kb);
+  GramPlan p = gram_plan(m, ka, kb, herm);
+  const size_t need = gram_work_bytes(p, compute);
+  if (!work || work_bytes < need)
+    return fail(FHIP_E_WORKSPACE, "fhip_gram: workspace of %zu bytes is smaller than the %zu needed",
+                work_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  const int ika = (int)ka, ikb = (int)kb;
+  switch (type) {
+    case FHIP_C64:
+      if (compute == FHIP_C32)
+        return dispatch_gram<double, float, double, true>(p, C, ldc, A, lda, B, ldb, m, ika, ikb, herm,
+                                                          alpha_re, alpha_im, beta_re, beta_im, work, s);
+      return dispatch_gram<double, double, double, true>(p, C, ldc, A, lda, B, ldb, m, ika, ikb, herm,
+                                                         alpha_re, alpha_im, beta_re, beta_im, work, s);
+    case FHIP_C32:
+      return dispatch_gram<float, float, float, true>(p, C, ldc, A, lda, B, ldb, m, ika, ikb, herm, alpha_re,
+                                                      alpha_im, beta_re, beta_im, work, s);
+    case FHIP_F64:
+      return dispatch_gram<double, double, double, false>(p, C, ldc, A, lda, B, ldb, m, ika, ikb, herm,
+                                                          alpha_re, alpha_im, beta_re, beta_im, work, s);
+    case FHIP_F32:
+      return dispatch_gram<float, float, float, false>(p, C, ldc, A, lda, B, ldb, m, ika, ikb, herm,
+                                                       alpha_re, alpha_im, beta_re, beta_im, work, s);
+  }
+  return fail(FHIP_E_UNSUPPORTED, "fhip_gram: unknown type %d", type);
+}

@@ -1,0 +1,304 @@
+"""Panel BLAS: gram, panelmul, cholqr2, rightmul and project
+(src/panelblas.jl:1-301), on the sweep pipeline and the HIP kernels.
+
+The step structure is the reference's: gram and the triangular solve sweep
+row blocks of height floor(N w / k); project runs p (p + 1) panel steps
+labelled (applying, i, j); cholqr2 is two passes, and a third when a shifted
+Cholesky fired. The k x k Cholesky factorizations run on the host in LAPACK,
+as in the reference (src/panelblas.jl:148-155).
+"""
+import numpy as np
+import scipy.linalg
+
+from . import plan as _plan
+from .device_api import ArgumentError
+from .executor import PanelSteps, RowSteps, readpanels, rowtraversal, sweep, updatepanels, writepanels
+from .operator import apply_operator, assert_operator_shape, assert_square_operator
+from .panels import (assert_conformal, assert_same_panels, assert_same_rows, npanels, panelrange, panelwidth)
+
+
+def shared_plan(X, Y):
+    """src/panelblas.jl:4-8."""
+    if X.dtype != Y.dtype:
+        raise ArgumentError(f"panel matrices in one operation must share their compute eltype, got {X.dtype} and "
+                            f"{Y.dtype}. Narrowing belongs on the storage tiers, not here")
+    if X.plan is not Y.plan:
+        raise ArgumentError("panel matrices in one operation must belong to the same ResidencyPlan, since the plan "
+                            "is what owns the staging buffers the operation runs through")
+    return X.plan
+
+
+def carry(step):
+    """The first step overwrites the accumulator, later ones add (src/panelblas.jl:55-59)."""
+    return 0.0 if step == 0 else 1.0
+
+
+def collect_small(device, plan):
+    """src/panelblas.jl:61-65."""
+    host = np.empty(device.shape, dtype=device.dtype, order="F")
+    plan.backend.d2h(host, device)
+    return host
+
+
+def gram(X, Y=None, *, nbuffers=None, compute=None):
+    """The k x k matrix X^H Y, accumulated on the device one row block at a
+    time and copied back once (src/panelblas.jl:10-53). `compute=np.complex64`
+    forms it on the FP32 matrix cores with FP64 sums across row chunks."""
+    same = Y is None or Y is X
+    Y = X if Y is None else Y
+    plan = shared_plan(X, Y)
+    assert_conformal(X, Y)
+    T = X.dtype
+    backend = plan.backend
+    traversal = rowtraversal(X)
+    height = traversal.blocks[0][1] - traversal.blocks[0][0]
+    accumulator = _plan.checkout_device_buffers(plan, T, (X.k, X.k), 1)[0]
+    work = None
+    try:
+        work = _plan.checkout_workspace(
+            plan, backend.gram_workspace_bytes(height, X.k, X.k, T, hermitian=same, compute=compute))
+        if same:
+            def step(block, Xb):
+                backend.gram_accumulate(accumulator, Xb, Xb, 1.0, carry(block), work, compute)
+            sweep(step, readpanels(X), nbuffers=nbuffers, traversal=traversal)
+        else:
+            def step(block, Xb, Yb):
+                backend.gram_accumulate(accumulator, Xb, Yb, 1.0, carry(block), work, compute)
+            sweep(step, readpanels(X), readpanels(Y), nbuffers=nbuffers, traversal=traversal)
+        return collect_small(accumulator, plan)
+    finally:
+        _plan.checkin_device_buffers(plan, [accumulator, work])
+
+
+def panelmul(Y, G, X, *, nbuffers=None):
+    """Y_j <- G X_j for every column panel j (src/panelblas.jl:67-100)."""
+    shared_plan(X, Y)
+    if X is Y:
+        raise ArgumentError("panelmul cannot write its output over its input: the operator is applied panel by "
+                            "panel and would read columns it has already overwritten")
+    assert_same_panels(X, Y)
+    assert_operator_shape(G, Y.N, X.N)
+    sweep(lambda _, Xb, Yb: apply_operator(Yb, G, Xb), readpanels(X), writepanels(Y), nbuffers=nbuffers)
+    return Y
+
+
+def cholqr2(Y, *, nbuffers=None):
+    """Orthonormalizes Y in place by CholeskyQR2 and returns the k x k upper
+    triangular R with Q R equal to the original Y (src/panelblas.jl:102-139).
+    A pass whose Gram needed a shift triggers a third pass (shifted
+    CholeskyQR3, Fukaya et al.)."""
+    R, shifted = cholqr_pass(Y, nbuffers)
+    R2, shifted2 = cholqr_pass(Y, nbuffers)
+    R = R2 @ R
+    if shifted or shifted2:
+        R3, _ = cholqr_pass(Y, nbuffers)
+        R = R3 @ R
+    return R
+
+
+def cholqr_pass(Y, nbuffers=None):
+    """src/panelblas.jl:141-146."""
+    G = gram(Y, nbuffers=nbuffers)
+    R, shifted = cholesky_upper(G, Y.N, Y.k)
+    rdiv_rows(Y, R, nbuffers=nbuffers)
+    return R, shifted
+
+
+def _potrf(G):
+    """Upper Cholesky factor of the Hermitian matrix whose upper triangle G
+    holds (LAPACK potrf, as cholesky(Hermitian(G)) does); None if it fails."""
+    potrf, = scipy.linalg.get_lapack_funcs(("potrf",), (G,))
+    c, info = potrf(G, lower=False, clean=True, overwrite_a=False)
+    if info != 0:
+        return None
+    return np.asfortranarray(np.triu(c))
+
+
+def cholesky_upper(G, N, k):
+    """The upper Cholesky factor of G, retried with G + s I,
+    s = 11 N k eps ||G||_F, when G is not numerically positive definite
+    (src/panelblas.jl:148-155). Returns (R, shifted)."""
+    G = np.asarray(G)
+    plain = _potrf(G)
+    if plain is not None:
+        return plain, False
+    eps = np.finfo(G.dtype).eps
+    shift = 11 * N * k * eps * np.linalg.norm(G)
+    shifted = _potrf(G + shift * np.eye(G.shape[0], dtype=G.dtype))
+    if shifted is None:
+        raise ArgumentError(
+            f"the {k}×{k} Gram matrix is not positive definite even after a shift of {shift}, so the columns are "
+            f"numerically dependent: a zero column, a duplicated one, or a condition number past {1 / eps}. Drop "
+            "the dependent columns before orthonormalizing")
+    return shifted, True
+
+
+def rdiv_rows(Y, R, *, nbuffers=None, compute=None):
+    """Y <- Y R^-1 over row blocks (src/panelblas.jl:157-169)."""
+    plan = Y.plan
+    T = Y.dtype
+    backend = plan.backend
+    factor = _plan.checkout_device_buffers(plan, T, R.shape, 1)[0]
+    work = None
+    try:
+        work = _plan.checkout_workspace(plan, backend.trsm_workspace_bytes(Y.k, T if compute is None else compute))
+        backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
+        sweep(lambda _, Yb: backend.rdiv_upper(Yb, factor, work, compute), updatepanels(Y), nbuffers=nbuffers,
+              traversal=rowtraversal(Y))
+    finally:
+        _plan.checkin_device_buffers(plan, [factor, work])
+    return Y
+
+
+def rightmul(*args, nbuffers=None):
+    """rightmul(Y, C): Y <- Y C in place (C k x k); rightmul(dest, src, C):
+    dest <- src C (src/panelblas.jl:171-245)."""
+    if len(args) == 2:
+        return _rightmul_inplace(args[0], args[1], nbuffers)
+    if len(args) == 3:
+        return _rightmul_into(args[0], args[1], args[2], nbuffers)
+    raise TypeError("rightmul takes (Y, C) or (dest, src, C)")
+
+
+def _rightmul_inplace(Y, C, nbuffers):
+    C = np.asarray(C)
+    if C.shape != (Y.k, Y.k):
+        raise ArgumentError(f"rightmul(Y, C) writes the product back into Y, which has {Y.k} columns and cannot grow "
+                            f"or shrink, so C must be {Y.k}×{Y.k}, got {C.shape}. Use rightmul(dest, Y, C) when the "
+                            "column count changes")
+    from .panels import assert_writable
+    assert_writable(Y)
+    plan = Y.plan
+    T = Y.dtype
+    backend = plan.backend
+    traversal = rowtraversal(Y)
+    from .executor import buffer_dims
+    factor = _plan.checkout_device_buffers(plan, T, (Y.k, Y.k), 1)[0]
+    scratch = _plan.checkout_device_buffers(plan, T, buffer_dims(traversal, Y), 1)[0]
+    try:
+        backend.h2d(factor, np.asfortranarray(C.astype(T)))
+
+        def step(_, Yb):
+            # The last block may be shorter than the buffer (src/panelblas.jl:212-216).
+            product = scratch.view((0, Yb.rows), None)
+            backend.rightmul_gemm(product, Yb, factor)
+            backend.convert_copy(Yb, product)
+        sweep(step, updatepanels(Y), nbuffers=nbuffers, traversal=traversal)
+    finally:
+        _plan.checkin_device_buffers(plan, [factor, scratch])
+    return Y
+
+
+def _rightmul_into(dest, src, C, nbuffers):
+    if dest is src:
+        raise ArgumentError("rightmul(dest, src, C) needs two different matrices, since one of them streams up while "
+                            "the other streams down. Use rightmul(Y, C) when the product goes back where it came from")
+    C = np.asarray(C)
+    if C.shape != (src.k, dest.k):
+        raise ArgumentError(f"rightmul(dest, src, C) computes src C, so C must be {src.k}×{dest.k} to take a "
+                            f"{src.N}×{src.k} source into a {dest.N}×{dest.k} destination, got {C.shape}")
+    plan = shared_plan(src, dest)
+    assert_same_rows(src, dest)
+    T = dest.dtype
+    backend = plan.backend
+    # The shorter of the two row-block heights serves both (src/panelblas.jl:230-234).
+    height = min(_plan.row_block_height(src.N, src.k, src.w), _plan.row_block_height(dest.N, dest.k, dest.w))
+    traversal = RowSteps(src.N, height)
+    factor = _plan.checkout_device_buffers(plan, T, (src.k, dest.k), 1)[0]
+    try:
+        backend.h2d(factor, np.asfortranarray(C.astype(T)))
+        sweep(lambda _, sb, db: backend.rightmul_gemm(db, sb, factor), readpanels(src), writepanels(dest),
+              nbuffers=nbuffers, traversal=traversal)
+    finally:
+        _plan.checkin_device_buffers(plan, [factor])
+    return dest
+
+
+def project_schedule(p):
+    """The panels and labels of project's sweep: for each j, apply G to panel j,
+    then reduce every panel i against it (src/panelblas.jl:272-281)."""
+    panels, labels = [], []
+    for j in range(p):
+        panels.append(j)
+        labels.append((True, j, j))
+        for i in range(p):
+            panels.append(i)
+            labels.append((False, i, j))
+    return panels, labels
+
+
+def project(Q, G, *, nbuffers=None):
+    """S = Q^H G Q without storing G Q (src/panelblas.jl:247-301)."""
+    plan = Q.plan
+    assert_square_operator(G, Q.N)
+    p = npanels(Q)
+    T = Q.dtype
+    backend = plan.backend
+    panels, labels = project_schedule(p)
+    image = _plan.checkout_device_buffers(plan, T, (Q.N, Q.w), 1)[0]
+    accumulator = None
+    work = None
+    try:
+        accumulator = _plan.checkout_device_buffers(plan, T, (Q.k, Q.k), 1)[0]
+        work = _plan.checkout_workspace(plan, backend.gram_workspace_bytes(Q.N, Q.w, Q.w, T, hermitian=False))
+
+        def step(label, Qb):
+            applying, i, j = label
+            GQj = image.view(None, (0, panelwidth(Q, j)))
+            if applying:
+                apply_operator(GQj, G, Qb)
+            else:
+                ri, rj = panelrange(Q, i), panelrange(Q, j)
+                block = accumulator.view((ri.start, ri.stop), (rj.start, rj.stop))
+                backend.gram_accumulate(block, Qb, GQj, 1.0, 0.0, work)
+        sweep(step, readpanels(Q), nbuffers=nbuffers, traversal=PanelSteps(panels, labels))
+        return collect_small(accumulator, plan)
+    finally:
+        _plan.checkin_device_buffers(plan, [image, accumulator, work])
+
+
+UNIT_ROUNDOFF_FP32 = 2.0 ** -24
+
+
+def cholesky_upper_shifted_fp32(G, N, k):
+    """The first pass's preconditioning factor: Cholesky in FP32 of
+    G + s I with s = 11 (N k + k (k + 1)) u32 ||Y||_2^2, ||Y||_2^2 the largest
+    eigenvalue of the Gram (shifted CholeskyQR, Fukaya et al. 2020). This has
+    no counterpart in the reference, whose shift rule (src/panelblas.jl:151)
+    is for an FP64 Gram that failed outright."""
+    G = np.asarray(G)
+    lam = scipy.linalg.eigh(G, eigvals_only=True, subset_by_index=[k - 1, k - 1])[0]
+    shift = 11.0 * (N * k + k * (k + 1)) * UNIT_ROUNDOFF_FP32 * lam
+    G32 = (G + shift * np.eye(k)).astype(np.complex64 if np.iscomplexobj(G) else np.float32)
+    R = _potrf(G32)
+    if R is None:
+        raise ArgumentError(
+            f"the {k}×{k} FP32 Gram matrix is not positive definite after the shift {shift}; the columns are "
+            "numerically dependent")
+    return R, shift
+
+
+def cholqr_pass_mixed(Y, nbuffers=None):
+    """The mixed first pass: FP32 matrix-core Gram with FP64 sums across row
+    chunks, shifted FP32 Cholesky, FP32 triangular solve on the FP64 data."""
+    narrow = np.complex64 if np.issubdtype(Y.dtype, np.complexfloating) else np.float32
+    if Y.dtype not in (np.dtype(np.complex128),):
+        raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
+    G = gram(Y, nbuffers=nbuffers, compute=narrow)
+    R32, _ = cholesky_upper_shifted_fp32(G, Y.N, Y.k)
+    R = R32.astype(Y.dtype)
+    rdiv_rows(Y, R, nbuffers=nbuffers, compute=narrow)
+    return R
+
+
+def cholqr2_mixed(Y, *, nbuffers=None):
+    """cholqr2 with the first pass in mixed precision (FP32 Gram and solve,
+    shifted) as a preconditioner and the second pass in FP64. A third FP64
+    pass runs when the second one's Cholesky needed the reference's shift."""
+    R1 = cholqr_pass_mixed(Y, nbuffers)
+    R2, shifted = cholqr_pass(Y, nbuffers)
+    R = R2 @ R1
+    if shifted:
+        R3, _ = cholqr_pass(Y, nbuffers)
+        R = R3 @ R
+    return R

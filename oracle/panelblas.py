@@ -1,0 +1,134 @@
+"""The reference CPU backend's panel BLAS in its own step order, in FP64
+numpy + LAPACK.
+
+TEST INFRASTRUCTURE (see oracle/__init__.py). The reference CPU backend's
+"device" is a plain Array and its kernels are LinearAlgebra calls
+(src/kernels.jl:40, :73, :108; src/device_api.jl:291-299), so restating it
+means the same products over the same row blocks in the same order.
+"""
+import numpy as np
+import scipy.linalg
+
+from . import plan as P
+
+
+def gram(A, B=None, w=None):
+    """src/panelblas.jl:33-59: sum over row blocks of height floor(N w / k) of
+    A_b^H B_b, the first block overwriting (carry 0), later ones adding."""
+    B = A if B is None else B
+    N, k = A.shape
+    w = k if w is None else w
+    acc = None
+    for (r0, r1) in P.row_blocks(N, P.row_block_height(N, k, w)):
+        prod = A[r0:r1].conj().T @ B[r0:r1]
+        acc = prod if acc is None else acc + prod
+    return acc
+
+
+def cholesky_upper(G, N, k):
+    """src/panelblas.jl:148-155: upper factor of Hermitian(G) (upper triangle
+    read), retried with G + s I, s = 11 N k eps ||G||_F. Returns (R, shifted);
+    raises ValueError where the reference throws ArgumentError."""
+    G = np.asarray(G)
+    potrf, = scipy.linalg.get_lapack_funcs(("potrf",), (G,))
+    c, info = potrf(G, lower=False, clean=True)
+    if info == 0:
+        return np.triu(c), False
+    eps = np.finfo(G.dtype).eps
+    shift = 11 * N * k * eps * np.linalg.norm(G)
+    c, info = potrf(G + shift * np.eye(G.shape[0], dtype=G.dtype), lower=False, clean=True)
+    if info != 0:
+        raise ValueError("Gram matrix not positive definite even after the shift")
+    return np.triu(c), True
+
+
+def rdiv_upper(Y, R):
+    """src/kernels.jl:108: Y R^-1 with R upper (right side, no transpose)."""
+    # Y R^-1 = (R^-T Y^T)^T: solve R^T X^T = Y^T with R^T lower.
+    return scipy.linalg.solve_triangular(R, Y.T, trans="T", lower=False).T
+
+
+def rdiv_rows(Y, R, w=None):
+    """src/panelblas.jl:157-169: the solve block by block over the row traversal."""
+    N, k = Y.shape
+    w = k if w is None else w
+    out = np.empty_like(Y)
+    for (r0, r1) in P.row_blocks(N, P.row_block_height(N, k, w)):
+        out[r0:r1] = rdiv_upper(Y[r0:r1], R)
+    return out
+
+
+def cholqr_pass(Y, w=None):
+    """src/panelblas.jl:141-146."""
+    G = gram(Y, w=w)
+    R, shifted = cholesky_upper(G, Y.shape[0], Y.shape[1])
+    return rdiv_rows(Y, R, w), R, shifted
+
+
+def cholqr2(Y, w=None):
+    """src/panelblas.jl:130-139: returns (Q, R)."""
+    Q, R, s1 = cholqr_pass(Y, w)
+    Q, R2, s2 = cholqr_pass(Q, w)
+    R = R2 @ R
+    if s1 or s2:
+        Q, R3, _ = cholqr_pass(Q, w)
+        R = R3 @ R
+    return Q, R
+
+
+def rightmul(A, C):
+    """src/kernels.jl:73 over row blocks: A C."""
+    return A @ C
+
+
+def project(Q, apply, w):
+    """src/panelblas.jl:268-301: S = Q^H G Q in project's p (p + 1)-step
+    schedule; `apply(X)` is G applied to a block of columns."""
+    N, k = Q.shape
+    ranges = P.panel_ranges(k, w)
+    panels, labels = P.project_labels(len(ranges))
+    S = np.full((k, k), np.nan, dtype=Q.dtype)
+    image = None
+    for panel, (applying, i, j) in zip(panels, labels):
+        c0, c1 = ranges[panel]
+        Qb = Q[:, c0:c1]
+        if applying:
+            image = apply(Qb)
+        else:
+            ri, rj = ranges[i], ranges[j]
+            S[ri[0]:ri[1], rj[0]:rj[1]] = Qb.conj().T @ image
+    return S
+
+
+def panelmul(apply, X, w):
+    """src/panelblas.jl:91-100: G applied panel by panel."""
+    out = None
+    for (c0, c1) in P.panel_ranges(X.shape[1], w):
+        Yb = apply(X[:, c0:c1])
+        if out is None:
+            out = np.empty((Yb.shape[0], X.shape[1]), dtype=Yb.dtype)
+        out[:, c0:c1] = Yb
+    return out
+
+
+def orthogonality_error(Q):
+    """||Q^H Q - I||_F."""
+    k = Q.shape[1]
+    return np.linalg.norm(Q.conj().T @ Q - np.eye(k))
+
+
+def blastol(dtype, N, k):
+    """test/setup.jl:170: 100 eps sqrt(N k)."""
+    return 100 * np.finfo(dtype).eps * np.sqrt(N * k)
+
+
+def orthogonality_bound(dtype, N, k):
+    """test/setup.jl:172: 100 N k eps."""
+    return 100 * N * k * np.finfo(dtype).eps
+
+
+def isapprox(x, y, rtol):
+    """Julia's isapprox on arrays: ||x - y|| <= rtol max(||x||, ||y||)."""
+    x = np.asarray(x)
+    y = np.asarray(y)
+    return np.linalg.norm(x - y) <= rtol * max(np.linalg.norm(x), np.linalg.norm(y))

@@ -1,0 +1,241 @@
+"""The HIP kernels against the oracle, through the C ABI (src/kernels.jl
+semantics; test/test_kernels.jl is the model)."""
+import numpy as np
+import pytest
+
+from oracle import ghost as OG
+from oracle import operators as OO
+from oracle import panelblas as OB
+
+pytestmark = pytest.mark.gpu
+
+EL = [np.complex128, np.complex64, np.float64, np.float32]
+
+
+def randmatrix(rng, N, k, dtype):
+    A = rng.random((N, k)) - 0.5
+    if np.issubdtype(dtype, np.complexfloating):
+        A = A + 1j * (rng.random((N, k)) - 0.5)
+    return np.asfortranarray(A.astype(dtype))
+
+
+def tol(dtype, N, k):
+    return OB.blastol(dtype, N, k)
+
+
+@pytest.mark.parametrize("T", EL)
+def test_gram_accumulate(backend, T):
+    rng = np.random.default_rng(11)
+    A, B = randmatrix(rng, 12, 5, T), randmatrix(rng, 12, 4, T)
+    Ad, Bd = backend.todevice(A), backend.todevice(B)
+    C = backend.alloc_device(T, (5, 4))
+    backend.gram_accumulate(C, Ad, Bd, 1.0, 0.0)
+    assert OB.isapprox(backend.to_host(C), A.conj().T @ B, tol(T, 12, 5))
+    backend.gram_accumulate(C, Ad, Bd, 1.0, 1.0)
+    assert OB.isapprox(backend.to_host(C), 2 * (A.conj().T @ B), tol(T, 12, 5))
+    # A zero beta overwrites whatever the buffer held, NaN included.
+    backend.fill(C, np.nan)
+    backend.gram_accumulate(C, Ad, Bd, 1.0, 0.0)
+    assert OB.isapprox(backend.to_host(C), A.conj().T @ B, tol(T, 12, 5))
+    # Views: a row block and a sub-block of the accumulator.
+    Cf = backend.alloc_device(T, (5, 4))
+    backend.fill(Cf, 0)
+    backend.gram_accumulate(Cf.view((0, 3), (0, 2)), Ad.view((0, 6), (0, 3)), Bd.view((0, 6), (0, 2)), 1.0, 0.0)
+    H = backend.to_host(Cf)
+    assert OB.isapprox(H[:3, :2], A[:6, :3].conj().T @ B[:6, :2], tol(T, 6, 3))
+    assert np.all(H[3:, :] == 0) and np.all(H[:, 2:] == 0)
+
+
+@pytest.mark.parametrize("T", [np.complex128, np.float64])
+@pytest.mark.parametrize("m,k", [(1, 1), (40, 23), (1000, 64), (5000, 100), (3001, 200), (257, 33)])
+def test_gram_shapes_and_hermitian_exactness(backend, T, m, k):
+    rng = np.random.default_rng(m * 1000 + k)
+    A = randmatrix(rng, m, k, T)
+    Ad = backend.todevice(A)
+    C = backend.alloc_device(T, (k, k))
+    backend.gram_accumulate(C, Ad, Ad, 1.0, 0.0)
+    H = backend.to_host(C)
+    assert OB.isapprox(H, A.conj().T @ A, tol(T, m, k))
+    assert np.array_equal(H, H.conj().T)
+    # A different B (general tiles) on the same data agrees to tolerance.
+    Bd = backend.todevice(A.copy(order="F"))
+    C2 = backend.alloc_device(T, (k, k))
+    backend.gram_accumulate(C2, Ad, Bd, 1.0, 0.0)
+    assert OB.isapprox(backend.to_host(C2), H, tol(T, m, k))
+
+
+def test_gram_complex_alpha_beta(backend):
+    rng = np.random.default_rng(12)
+    T = np.complex128
+    A, B = randmatrix(rng, 300, 70, T), randmatrix(rng, 300, 70, T)
+    C0 = randmatrix(rng, 70, 70, T)
+    Ad, Bd, Cd = backend.todevice(A), backend.todevice(B), backend.todevice(C0)
+    al, be = 0.5 - 1.25j, -0.75 + 0.5j
+    backend.gram_accumulate(Cd, Ad, Bd, al, be)
+    assert OB.isapprox(backend.to_host(Cd), al * (A.conj().T @ B) + be * C0, tol(T, 300, 70))
+    # Same operand, complex alpha and beta: the lower triangle is still right.
+    Cd = backend.todevice(C0)
+    backend.gram_accumulate(Cd, Ad, Ad, al, be)
+    assert OB.isapprox(backend.to_host(Cd), al * (A.conj().T @ A) + be * C0, tol(T, 300, 70))
+
+
+def test_gram_is_deterministic(backend):
+    rng = np.random.default_rng(13)
+    A = randmatrix(rng, 20000, 96, np.complex128)
+    Ad = backend.todevice(A)
+    outs = []
+    for _ in range(3):
+        C = backend.alloc_device(np.complex128, (96, 96))
+        backend.gram_accumulate(C, Ad, Ad, 1.0, 0.0)
+        outs.append(backend.to_host(C))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+
+
+def test_gram_mixed_precision(backend):
+    rng = np.random.default_rng(14)
+    A = randmatrix(rng, 4096, 80, np.complex128)
+    Ad = backend.todevice(A)
+    C = backend.alloc_device(np.complex128, (80, 80))
+    backend.gram_accumulate(C, Ad, Ad, 1.0, 0.0, compute=np.complex64)
+    ref = A.conj().T @ A
+    err = np.linalg.norm(backend.to_host(C) - ref) / np.linalg.norm(ref)
+    assert err < 1e-5
+    assert err > 1e-12  # it really ran in FP32
+
+
+@pytest.mark.parametrize("T", EL)
+def test_rightmul_gemm(backend, T):
+    rng = np.random.default_rng(15)
+    A, C = randmatrix(rng, 12, 5, T), randmatrix(rng, 5, 4, T)
+    Ad, Cd = backend.todevice(A), backend.todevice(C)
+    dst = backend.alloc_device(T, (12, 4))
+    backend.fill(dst, np.nan)
+    backend.rightmul_gemm(dst, Ad, Cd)
+    assert OB.isapprox(backend.to_host(dst), A @ C, tol(T, 12, 5))
+    backend.fill(dst, 0)
+    backend.rightmul_gemm(dst.view((0, 7), None), Ad.view((0, 7), None), Cd)
+    H = backend.to_host(dst)
+    assert OB.isapprox(H[:7], A[:7] @ C, tol(T, 7, 5))
+    assert np.all(H[7:] == 0)
+
+
+@pytest.mark.parametrize("m,s,r", [(1000, 256, 256), (777, 70, 33), (64, 1, 1)])
+def test_rightmul_gemm_shapes(backend, m, s, r):
+    rng = np.random.default_rng(m + s + r)
+    A, C = randmatrix(rng, m, s, np.complex128), randmatrix(rng, s, r, np.complex128)
+    dst = backend.alloc_device(np.complex128, (m, r))
+    backend.rightmul_gemm(dst, backend.todevice(A), backend.todevice(C))
+    assert OB.isapprox(backend.to_host(dst), A @ C, tol(np.complex128, m, s))
+
+
+@pytest.mark.parametrize("T", EL)
+def test_rdiv_upper(backend, T):
+    rng = np.random.default_rng(16)
+    Y = randmatrix(rng, 7, 5, T)
+    R = np.asfortranarray(np.triu(randmatrix(rng, 5, 5, T)) + 4 * np.eye(5, dtype=T))
+    Rd = backend.todevice(R)
+    Yd = backend.todevice(Y)
+    backend.rdiv_upper(Yd, Rd)
+    assert OB.isapprox(backend.to_host(Yd), OB.rdiv_upper(Y, R), tol(T, 7, 5))
+    # A row-strided view: rows outside it are untouched.
+    S = backend.todevice(Y)
+    backend.rdiv_upper(S.view((0, 3), None), Rd)
+    H = backend.to_host(S)
+    assert OB.isapprox(H[:3], OB.rdiv_upper(Y[:3], R), tol(T, 3, 5))
+    assert np.array_equal(H[3:], Y[3:])
+
+
+@pytest.mark.parametrize("m,k", [(1000, 64), (5000, 512), (333, 45), (100, 1), (200, 96)])
+def test_rdiv_upper_shapes(backend, m, k):
+    rng = np.random.default_rng(m + k)
+    T = np.complex128
+    Y = randmatrix(rng, m, k, T)
+    G = Y.conj().T @ Y
+    R = np.asfortranarray(np.linalg.cholesky(G).conj().T)
+    Yd = backend.todevice(Y)
+    backend.rdiv_upper(Yd, backend.todevice(R))
+    X = backend.to_host(Yd)
+    ref = OB.rdiv_upper(Y, R)
+    assert OB.isapprox(X, ref, 1e-12)
+    assert np.linalg.norm(X @ R - Y) <= 1e-13 * np.linalg.norm(Y) * np.sqrt(k)
+
+
+def test_rdiv_upper_mixed(backend):
+    rng = np.random.default_rng(17)
+    Y = randmatrix(rng, 2000, 100, np.complex128)
+    R = np.asfortranarray(np.linalg.cholesky(Y.conj().T @ Y).conj().T)
+    Yd = backend.todevice(Y)
+    backend.rdiv_upper(Yd, backend.todevice(R), compute=np.complex64)
+    X = backend.to_host(Yd)
+    err = np.linalg.norm(X - OB.rdiv_upper(Y, R)) / np.linalg.norm(X)
+    assert 1e-12 < err < 1e-4
+
+
+@pytest.mark.parametrize("T", EL)
+def test_convert_copy_and_fill(backend, T):
+    rng = np.random.default_rng(18)
+    A = randmatrix(rng, 9, 5, T)
+    src = backend.todevice(A)
+    dst = backend.alloc_device(T, (9, 5))
+    backend.convert_copy(dst, src)
+    assert np.array_equal(backend.to_host(dst), A)
+    S = np.complex64 if np.issubdtype(T, np.complexfloating) else np.float32
+    narrow = backend.alloc_device(S, (9, 5))
+    backend.convert_copy(narrow, src)
+    assert np.array_equal(backend.to_host(narrow), A.astype(S))
+    backend.fill(dst, 0)
+    backend.convert_copy(dst.view(None, (0, 2)), src.view(None, (3, 5)))
+    backend.convert_copy(dst.view((0, 3), (2, 5)), src.view((6, 9), (0, 3)))
+    H = backend.to_host(dst)
+    assert np.array_equal(H[:, :2], A[:, 3:5])
+    assert np.array_equal(H[:3, 2:5], A[6:9, :3])
+
+
+@pytest.mark.parametrize("T", EL)
+def test_ghost_fill_is_bit_exact(backend, T):
+    N, k, seed = 5000, 6, 0xDEADBEEF
+    d = backend.alloc_device(T, (N, k))
+    backend.ghost_fill(d, 0, 1, seed)
+    H = backend.to_host(d)
+    ref = OG.ghost(T, 0, N, 1, k, seed)
+    assert np.array_equal(H, ref)
+    # A row block and a column offset regenerate the same entries.
+    b = backend.alloc_device(T, (37, 2))
+    backend.ghost_fill(b, 1234, 4, seed)
+    assert np.array_equal(backend.to_host(b), ref[1234:1271, 3:5])
+
+
+def test_ghost_fill_large_bit_exact(backend):
+    # Enough draws that tail and wedge paths are taken many times.
+    N = 400000
+    d = backend.alloc_device(np.complex128, (N, 2))
+    backend.ghost_fill(d, 0, 7, 12345)
+    H = backend.to_host(d)
+    assert np.array_equal(H, OG.ghost(np.complex128, 0, N, 7, 2, 12345))
+    assert np.abs(H.real).max() * np.sqrt(2) > 3.7
+
+
+def test_stencils(backend):
+    import funicular_hip as F
+    rng = np.random.default_rng(19)
+    N = 1000
+    X = randmatrix(rng, N, 3, np.complex128)
+    diag = 1 + 0.5 * np.sin(2 * np.pi * np.arange(N) / N)
+    Xd = backend.todevice(X)
+    Yd = backend.alloc_device(np.complex128, (N, 3))
+    for dg in (None, diag):
+        G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, diag=dg, backend=backend)
+        G.apply(Yd, Xd)
+        assert np.allclose(backend.to_host(Yd), OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg), rtol=1e-14, atol=1e-14)
+        G.adjoint().apply(Yd, Xd)
+        assert np.allclose(backend.to_host(Yd), OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg, adjoint=True),
+                           rtol=1e-14, atol=1e-14)
+    n = (10, 4, 25)
+    H = F.Helmholtz3D(*n, OO.helmholtz_diagonal(), -1.0, backend=backend)
+    H.apply(Yd, Xd)
+    assert np.allclose(backend.to_host(Yd), OO.helmholtz_apply(X, n, OO.helmholtz_diagonal()), rtol=1e-14,
+                       atol=1e-14)
+    H.adjoint().apply(Yd, Xd)
+    assert np.allclose(backend.to_host(Yd), OO.helmholtz_apply(X, n, OO.helmholtz_diagonal(), adjoint=True),
+                       rtol=1e-14, atol=1e-14)
+    assert F.check_operator(H, seed=1)

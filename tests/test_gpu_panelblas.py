@@ -1,0 +1,297 @@
+"""Panel BLAS and the sweep pipeline on the HIP backend against the oracle
+(test/test_panelblas.jl, test/test_executor.jl, test/test_ghosts.jl are the
+models)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import funicular_hip as F
+from funicular_hip import plan as P
+from funicular_hip.executor import PanelSteps, RowSteps, readpanels, rowtraversal, sweep, updatepanels, writepanels
+from oracle import ghost as OG
+from oracle import operators as OO
+from oracle import panelblas as OB
+
+pytestmark = pytest.mark.gpu
+
+WIDTHS = (23, 12, 7, 1)
+C16 = np.complex128
+
+
+def randmatrix(rng, N, k, dtype=C16):
+    A = rng.random((N, k)) - 0.5
+    if np.issubdtype(dtype, np.complexfloating):
+        A = A + 1j * (rng.random((N, k)) - 0.5)
+    return np.asfortranarray(A.astype(dtype))
+
+
+def testplan(backend, **kw):
+    kw.setdefault("device_budget", 64 * 2 ** 20)
+    kw.setdefault("host_budget", 64 * 2 ** 20)
+    return F.ResidencyPlan(backend=backend, **kw)
+
+
+@pytest.mark.parametrize("T", [C16, np.complex64, np.float64])
+@pytest.mark.parametrize("w", WIDTHS)
+@pytest.mark.parametrize("nb", (1, 2))
+def test_gram_against_dense(backend, T, w, nb):
+    rng = np.random.default_rng(w * 10 + nb)
+    N, k = 40, 23
+    A, B = randmatrix(rng, N, k, T), randmatrix(rng, N, k, T)
+    plan = testplan(backend)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Y = F.PanelMatrix.from_array(B, plan=plan, w=w)
+    tol = OB.blastol(T, N, k)
+    assert OB.isapprox(F.gram(X, Y, nbuffers=nb), OB.gram(A, B, w), tol)
+    assert OB.isapprox(F.gram(X, nbuffers=nb), OB.gram(A, w=w), tol)
+    assert np.array_equal(F.to_matrix(X), A)
+
+
+@pytest.mark.parametrize("w", WIDTHS)
+def test_gram_same_bits_at_nbuffers_1_and_2(backend, w):
+    rng = np.random.default_rng(w)
+    A, B = randmatrix(rng, 40, 23), randmatrix(rng, 40, 23)
+    plan = testplan(backend)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Y = F.PanelMatrix.from_array(B, plan=plan, w=w)
+    assert np.array_equal(F.gram(X, Y, nbuffers=1), F.gram(X, Y, nbuffers=2))
+    assert np.array_equal(F.gram(X, nbuffers=1), F.gram(X, nbuffers=2))
+
+
+def test_gram_rejects_mismatched_operands(backend):
+    plan = testplan(backend)
+    X = F.PanelMatrix(10, 20, plan=plan, w=7)
+    with pytest.raises(F.ArgumentError):
+        F.gram(X, F.PanelMatrix(10, 20, plan=plan, w=5))
+    with pytest.raises(F.ArgumentError):
+        F.gram(X, F.PanelMatrix(10, 20, plan=plan, w=7, dtype=np.complex64))
+    with pytest.raises(F.ArgumentError):
+        F.gram(X, F.PanelMatrix(10, 20, plan=testplan(backend), w=7))
+
+
+@pytest.mark.parametrize("w", WIDTHS)
+@pytest.mark.parametrize("nb", (1, 2))
+def test_cholqr2_orthonormalizes(backend, w, nb):
+    rng = np.random.default_rng(100 + w)
+    N, k = 40, 23
+    A = randmatrix(rng, N, k)
+    plan = testplan(backend)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    R = F.cholqr2(Y, nbuffers=nb)
+    Q = F.to_matrix(Y)
+    assert R.shape == (k, k)
+    assert np.allclose(R, np.triu(R))
+    assert OB.orthogonality_error(Q) <= OB.orthogonality_bound(C16, N, k)
+    assert OB.isapprox(Q @ R, A, OB.blastol(C16, N, k))
+    Qo, Ro = OB.cholqr2(A, w)
+    assert OB.isapprox(Q, Qo, 1e-12) and OB.isapprox(R, Ro, 1e-12)
+
+
+@pytest.mark.parametrize("kappa", (1e2, 1e6, 1e10))
+def test_cholqr2_survives_conditioning(backend, kappa):
+    # test/test_panelblas.jl:284-325
+    rng = np.random.default_rng(int(np.log10(kappa)))
+    N, k, w = 40, 23, 7
+    U = scipy.linalg.qr(randmatrix(rng, N, k), mode="economic")[0]
+    V = scipy.linalg.qr(randmatrix(rng, k, k))[0]
+    A = np.asfortranarray(U @ np.diag(10.0 ** np.linspace(0, -np.log10(kappa), k)) @ V.conj().T)
+    plan = testplan(backend)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    R = F.cholqr2(Y)
+    Q = F.to_matrix(Y)
+    shifted = kappa > 1 / np.sqrt(np.finfo(float).eps)
+    bound = 1e-8 if shifted else OB.orthogonality_bound(C16, N, k)
+    assert OB.orthogonality_error(Q) <= bound
+    assert np.linalg.norm(Q @ R - A) <= 1e-10 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("w", WIDTHS)
+@pytest.mark.parametrize("nb", (1, 2))
+def test_project_and_panelmul_against_dense(backend, w, nb):
+    rng = np.random.default_rng(200 + w)
+    N, k = 40, 23
+    Gh = randmatrix(rng, N, N)
+    G = F.DenseOperator(Gh, backend)
+    A = randmatrix(rng, N, k)
+    plan = testplan(backend)
+    Q = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    tol = OB.blastol(C16, N, k)
+    S = F.project(Q, G, nbuffers=nb)
+    assert OB.isapprox(S, OB.project(A, lambda X: Gh @ X, w), tol)
+    assert np.array_equal(F.to_matrix(Q), A)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    F.panelmul(Y, G, Q, nbuffers=nb)
+    assert OB.isapprox(F.to_matrix(Y), OB.panelmul(lambda X: Gh @ X, A, w), tol)
+    Z = F.PanelMatrix(N, k, plan=plan, w=w)
+    F.panelmul(Z, G.adjoint(), Q, nbuffers=nb)
+    assert OB.isapprox(F.to_matrix(Z), Gh.conj().T @ A, tol)
+
+
+@pytest.mark.parametrize("w", WIDTHS)
+@pytest.mark.parametrize("nb", (1, 2))
+def test_rightmul_against_dense(backend, w, nb):
+    rng = np.random.default_rng(300 + w)
+    N, s = 40, 23
+    A = randmatrix(rng, N, s)
+    C = randmatrix(rng, s, 12)
+    D = randmatrix(rng, s, s)
+    plan = testplan(backend)
+    src = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    dest = F.PanelMatrix(N, 12, plan=plan, w=max(1, min(w - 3, 12)))
+    assert F.rightmul(dest, src, C, nbuffers=nb) is dest
+    assert OB.isapprox(F.to_matrix(dest), A @ C, OB.blastol(C16, N, s))
+    assert F.rightmul(src, D, nbuffers=nb) is src
+    assert OB.isapprox(F.to_matrix(src), A @ D, OB.blastol(C16, N, s))
+
+
+def test_rightmul_same_bits_at_nbuffers(backend):
+    rng = np.random.default_rng(301)
+    A, C = randmatrix(rng, 40, 23), randmatrix(rng, 23, 12)
+    out = []
+    for nb in (1, 2):
+        plan = testplan(backend)
+        src = F.PanelMatrix.from_array(A, plan=plan, w=7)
+        dest = F.PanelMatrix(40, 12, plan=plan, w=4)
+        out.append(F.to_matrix(F.rightmul(dest, src, C, nbuffers=nb)))
+    assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("w", (8, 7, 1))
+def test_ghost_regenerates_and_matches_oracle(backend, w):
+    N, k = 24, 8
+    plan = testplan(backend)
+    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    A = F.to_matrix(O)
+    assert np.array_equal(A, OG.matrix(C16, N, k, 0xDEADBEEF))
+    assert np.array_equal(A, F.to_matrix(F.GhostPanels(C16, N, k, plan=testplan(backend), seed=0xDEADBEEF, w=3)))
+    assert not np.array_equal(A, F.to_matrix(F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF + 1, w=w)))
+    assert 0.5 < np.linalg.norm(A) / np.sqrt(N * k) < 2
+    assert np.linalg.matrix_rank(A) == min(N, k)
+
+
+def test_ghost_feeds_sweeps_and_refuses_writes(backend):
+    rng = np.random.default_rng(400)
+    N, k, w = 40, 23, 7
+    plan = testplan(backend)
+    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    A = OG.matrix(C16, N, k, 0xDEADBEEF)
+    Gh = randmatrix(rng, N, N)
+    G = F.DenseOperator(Gh, backend)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    F.panelmul(Y, G, O)
+    assert OB.isapprox(F.to_matrix(Y), Gh @ A, OB.blastol(C16, N, k))
+    assert OB.isapprox(F.gram(O, Y), A.conj().T @ Gh @ A, OB.blastol(C16, N, k))
+    C = randmatrix(rng, k, 12)
+    dest = F.PanelMatrix(N, 12, plan=plan, w=4)
+    F.rightmul(dest, O, C)
+    assert OB.isapprox(F.to_matrix(dest), A @ C, OB.blastol(C16, N, k))
+    with pytest.raises(F.ArgumentError):
+        F.cholqr2(O)
+    with pytest.raises(F.ArgumentError):
+        F.panelmul(O, G, Y)
+    assert all(p.pins == 0 for p in O.panels)
+
+
+def test_sketch_chain_matches_the_oracle(backend):
+    # The configuration-1 chain at a small N: ghost -> tridiagonal -> cholqr2 -> project.
+    N, k, w = 4096, 64, 8
+    plan = testplan(backend)
+    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, backend=backend)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    F.panelmul(Y, G, O)
+    R = F.cholqr2(Y)
+    S = F.project(Y, G)
+    Om = OG.matrix(C16, N, k, 0xDEADBEEF)
+    Yo = OB.panelmul(lambda X: OO.tridiag_apply(X, 2 + 0.1j, -1.0), Om, w)
+    Qo, Ro = OB.cholqr2(Yo, w)
+    So = OB.project(Qo, lambda X: OO.tridiag_apply(X, 2 + 0.1j, -1.0), w)
+    Q = F.to_matrix(Y)
+    assert OB.orthogonality_error(Q) <= 1e-13
+    assert np.linalg.norm(Q @ R - Yo) / np.linalg.norm(Yo) <= 1e-13
+    assert OB.isapprox(S, So, 1e-13)
+
+
+@pytest.mark.parametrize("nb", (1, 2))
+def test_device_resident_plan(backend, nb):
+    rng = np.random.default_rng(500)
+    N, k, w = 3000, 96, 16
+    A = randmatrix(rng, N, k)
+    plan = testplan(backend, device_budget=256 * 2 ** 20, device_resident=True)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    assert Y.resident
+    assert OB.isapprox(F.gram(Y, nbuffers=nb), OB.gram(A, w=w), OB.blastol(C16, N, k))
+    R = F.cholqr2(Y, nbuffers=nb)
+    Q = F.to_matrix(Y)
+    assert OB.orthogonality_error(Q) <= 1e-13
+    assert np.linalg.norm(Q @ R - A) / np.linalg.norm(A) <= 1e-13
+    # Same bits as the streamed path: the kernels see the same row blocks.
+    hp = testplan(backend, device_budget=256 * 2 ** 20)
+    Yh = F.PanelMatrix.from_array(A, plan=hp, w=w)
+    Rh = F.cholqr2(Yh, nbuffers=nb)
+    assert np.array_equal(R, Rh)
+    assert np.array_equal(Q, F.to_matrix(Yh))
+
+
+@pytest.mark.parametrize("w", (23, 8, 7, 1))
+@pytest.mark.parametrize("nb", (1, 2))
+def test_sweeps_move_every_panel(backend, w, nb):
+    rng = np.random.default_rng(600 + w)
+    N, k = 9, 23
+    A = randmatrix(rng, N, k)
+    plan = testplan(backend)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    order = []
+    seen = {}
+
+    def look(j, panel):
+        order.append(j)
+        seen[j] = panel.shape
+    F.foreachpanel(look, X, write=False, nbuffers=nb)
+    assert order == list(range(F.npanels(X)))
+    assert all(seen[j] == (N, F.panelwidth(X, j)) for j in seen)
+
+    def scale(j, src, dst):
+        backend.convert_copy(dst, src)
+    sweep(scale, readpanels(X), writepanels(Y), nbuffers=nb)
+    assert np.array_equal(F.to_matrix(Y), A)
+    assert all(p.epoch == 1 and p.dirty for p in Y.panels)
+    assert all(p.epoch == 0 for p in X.panels)
+    before = P.device_bytes_allocated(plan)
+    sweep(scale, readpanels(X), writepanels(Y), nbuffers=nb)
+    assert P.device_bytes_allocated(plan) == before
+
+
+def test_row_sweep_and_failure_path(backend):
+    rng = np.random.default_rng(700)
+    N, k, w = 13, 23, 7
+    A = randmatrix(rng, N, k)
+    R = np.asfortranarray(np.triu(randmatrix(rng, k, k)) + 4 * np.eye(k))
+    plan = testplan(backend)
+    pm = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    factor = backend.todevice(R)
+    sweep(lambda _, b: backend.rdiv_upper(b, factor), updatepanels(pm), traversal=rowtraversal(pm))
+    assert OB.isapprox(F.to_matrix(pm), OB.rdiv_upper(A, R), OB.blastol(C16, N, k))
+    before = P.device_bytes_allocated(plan)
+
+    def boom(j, panel):
+        if j == 1:
+            raise RuntimeError("boom")
+    with pytest.raises(RuntimeError):
+        F.foreachpanel(boom, pm)
+    assert P.device_bytes_allocated(plan) == before
+    assert all(p.pins == 0 for p in pm.panels)
+    with pytest.raises(F.ArgumentError):
+        sweep(lambda *a: None, updatepanels(pm), traversal=PanelSteps([0, 1, 0], [0, 1, 2]))
+    sweep(lambda *a: None, readpanels(pm), traversal=PanelSteps([0, 1, 0], [0, 1, 2]))
+
+
+def test_narrowed_host_storage(backend):
+    rng = np.random.default_rng(800)
+    N, k, w = 40, 23, 7
+    A = randmatrix(rng, N, k)
+    plan = testplan(backend, host_eltype=np.complex64)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    stored = A.astype(np.complex64).astype(C16)
+    assert OB.isapprox(F.gram(X), stored.conj().T @ stored, OB.blastol(C16, N, k))

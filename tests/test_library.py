@@ -1,0 +1,64 @@
+"""The C-ABI library: it loads, it exports every entry point the header
+declares, and its argument checks answer without touching a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import funicular_hip
+from funicular_hip import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "funicular_hip.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(fhip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_binds():
+    assert set(declared()) == set(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in declared():
+        assert hasattr(lib, name), name
+    assert b"gfx950" in lib.fhip_version()
+
+
+def test_argument_errors_need_no_device():
+    lib = _lib.load()
+    n = ctypes.c_size_t()
+    assert lib.fhip_gram_workspace_bytes(ctypes.byref(n), 10, 4, 4, 7, 7, 1) == _lib.FHIP_E_UNSUPPORTED
+    assert b"no kernel" in lib.fhip_last_error()
+    assert lib.fhip_gram_workspace_bytes(ctypes.byref(n), -1, 4, 4, 3, 3, 1) == _lib.FHIP_E_ARGUMENT
+    assert lib.fhip_gram_workspace_bytes(ctypes.byref(n), 1000, 64, 64, 3, 3, 1) == 0
+    assert n.value > 0
+    assert lib.fhip_trsm_runn(None, 1, 1, 4, None, 4, 3, 1, None, 0, None) == _lib.FHIP_E_UNSUPPORTED
+    assert lib.fhip_trsm_runn(None, 1, 1, 4, None, 4, 3, 2, None, 0, None) == _lib.FHIP_E_ARGUMENT
+    assert lib.fhip_ghost_randn(None, 3, 10, 0, 10, 0, 1, 0, None) == _lib.FHIP_E_ARGUMENT
+    with pytest.raises(_lib.LibraryError) as e:
+        _lib.call("fhip_convert_copy", None, 0, 1, None, 3, 1, 1, 1, None)
+    assert "null pointer" in str(e.value)
+
+
+def test_workspace_depends_on_shape_only():
+    # The Gram's chunking is fixed by (m, ka, kb): the sums never depend on
+    # the workspace a caller happens to pass.
+    lib = _lib.load()
+    a, b = ctypes.c_size_t(), ctypes.c_size_t()
+    lib.fhip_gram_workspace_bytes(ctypes.byref(a), 629856, 512, 512, 3, 3, 1)
+    lib.fhip_gram_workspace_bytes(ctypes.byref(b), 629856, 512, 512, 3, 3, 1)
+    assert a.value == b.value
+    lib.fhip_gram_workspace_bytes(ctypes.byref(b), 629855, 512, 512, 3, 3, 1)
+    assert b.value <= a.value
+
+
+def test_ghost_tables_match_the_oracle():
+    from oracle import ghost as OG
+    out = ctypes.c_uint64()
+    _lib.call("fhip_ghost_tables_checksum", ctypes.byref(out))
+    assert out.value == OG.tables_checksum()

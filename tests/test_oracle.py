@@ -1,0 +1,182 @@
+"""Pins the oracle itself: against brute force, closed forms, textbook
+routines and the reference's known answers, so that a dropped term, a wrong
+sign or a transposed operand in the oracle fails here."""
+import math
+
+import mpmath
+import numpy as np
+import pytest
+import scipy.linalg
+from scipy import stats
+
+from oracle import ghost as OG
+from oracle import operators as OO
+from oracle import panelblas as OB
+from oracle import plan as OP
+
+
+def randmatrix(rng, N, k, dtype=np.complex128):
+    """test/setup.jl:148: uniform in [-1/2, 1/2] (both parts for complex)."""
+    A = rng.random((N, k)) - 0.5
+    if np.issubdtype(dtype, np.complexfloating):
+        A = A + 1j * (rng.random((N, k)) - 0.5)
+    return A.astype(dtype)
+
+
+def test_gram_matches_mpmath_brute_force():
+    rng = np.random.default_rng(1)
+    A = randmatrix(rng, 9, 4)
+    B = randmatrix(rng, 9, 3)
+    G = OB.gram(A, B, w=2)
+    for i in range(4):
+        for j in range(3):
+            exact = mpmath.fsum(mpmath.conj(mpmath.mpc(A[r, i])) * mpmath.mpc(B[r, j]) for r in range(9))
+            assert abs(complex(exact) - G[i, j]) < 1e-14
+    # A^H B, not A^T B nor B^H A: a transposed or unconjugated product fails.
+    assert not np.allclose(G, (A.T @ B))
+    assert not np.allclose(G, (B.conj().T @ A).T)
+
+
+def test_gram_sums_every_row_block():
+    rng = np.random.default_rng(2)
+    A = randmatrix(rng, 40, 23)
+    for w in (23, 12, 7, 1):
+        G = OB.gram(A, w=w)
+        assert OB.isapprox(G, A.conj().T @ A, OB.blastol(np.complex128, 40, 23))
+    # A ragged last block (N not a multiple of h) is still summed.
+    assert OP.row_blocks(40, OP.row_block_height(40, 23, 7))[-1][1] == 40
+
+
+def test_cholesky_shift_known_answers():
+    # test/test_panelblas.jl:327-345
+    rng = np.random.default_rng(3)
+    N, k = 40, 23
+    H = randmatrix(rng, N, k)
+    G = H.conj().T @ H
+    R, shifted = OB.cholesky_upper(G, N, k)
+    assert not shifted
+    assert np.allclose(R.conj().T @ R, G)
+    assert np.allclose(R, np.triu(R)) and np.all(np.diag(R).real > 0)
+    S = G.copy()
+    S[:, -1] = 0
+    S[-1, :] = 0
+    R, shifted = OB.cholesky_upper(S, N, k)
+    assert shifted
+    assert np.allclose(R, np.triu(R))
+    s = 11 * N * k * np.finfo(float).eps * np.linalg.norm(S)
+    assert np.allclose(R.conj().T @ R, S + s * np.eye(k))
+    with pytest.raises(ValueError):
+        OB.cholesky_upper(-G, N, k)
+
+
+def test_rdiv_is_the_right_upper_solve():
+    rng = np.random.default_rng(4)
+    Y = randmatrix(rng, 7, 5)
+    R = np.triu(randmatrix(rng, 5, 5)) + 4 * np.eye(5)
+    X = OB.rdiv_upper(Y, R)
+    assert np.allclose(X @ R, Y)
+    assert not np.allclose(np.linalg.solve(R, Y.T).T, X)  # a left solve would differ
+
+
+def test_cholqr2_invariants():
+    rng = np.random.default_rng(5)
+    A = randmatrix(rng, 40, 23)
+    for w in (23, 12, 7, 1):
+        Q, R = OB.cholqr2(A, w)
+        assert OB.orthogonality_error(Q) <= OB.orthogonality_bound(np.complex128, 40, 23)
+        assert OB.isapprox(Q @ R, A, OB.blastol(np.complex128, 40, 23))
+        assert np.allclose(R, np.triu(R))
+        # Same R as a Householder QR up to the signs of the diagonal.
+        Rh = scipy.linalg.qr(A, mode="r")[0][:23]
+        ph = np.diag(Rh) / np.abs(np.diag(Rh))
+        assert np.allclose(np.diag(ph.conj()) @ Rh, R, atol=1e-12)
+
+
+def test_cholqr2_shifted_third_pass():
+    # test/test_panelblas.jl:284-325: past kappa ~ eps^-1/2 the shift fires.
+    rng = np.random.default_rng(6)
+    N, k = 40, 23
+    U = scipy.linalg.qr(randmatrix(rng, N, k), mode="economic")[0]
+    V = scipy.linalg.qr(randmatrix(rng, k, k))[0]
+    A = U @ np.diag(10.0 ** np.linspace(0, -10, k)) @ V.conj().T
+    Q, R = OB.cholqr2(A, 7)
+    assert OB.orthogonality_error(Q) <= 1e-8
+    assert np.linalg.norm(Q @ R - A) <= 1e-10 * np.linalg.norm(A)
+
+
+def test_project_against_explicit_product():
+    rng = np.random.default_rng(7)
+    N, k = 40, 23
+    G = randmatrix(rng, N, N)
+    Q = randmatrix(rng, N, k)
+    for w in (23, 12, 7, 1):
+        S = OB.project(Q, lambda X: G @ X, w)
+        assert OB.isapprox(S, Q.conj().T @ G @ Q, OB.blastol(np.complex128, N, k))
+        assert not np.isnan(S).any()  # every block assigned exactly once
+
+
+def test_tridiagonal_against_dense():
+    rng = np.random.default_rng(8)
+    N = 17
+    X = randmatrix(rng, N, 3)
+    diag = 1 + 0.5 * np.sin(2 * np.pi * np.arange(N) / N)
+    for dg in (None, diag):
+        D = OO.tridiag_dense(N, 2 + 0.1j, -1.0, dg)
+        assert np.allclose(OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg), D @ X)
+        assert np.allclose(OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg, adjoint=True), D.conj().T @ X)
+
+
+def test_helmholtz_against_dense():
+    rng = np.random.default_rng(9)
+    n = (4, 3, 5)
+    d = OO.helmholtz_diagonal()
+    assert abs(d - (5.91 - 0.0045j)) < 1e-15
+    X = randmatrix(rng, 60, 2)
+    D = OO.helmholtz_dense(n, d)
+    assert np.allclose(OO.helmholtz_apply(X, n, d), D @ X)
+    assert np.allclose(OO.helmholtz_apply(X, n, d, adjoint=True), D.conj().T @ X)
+
+
+def test_splitmix_published_outputs():
+    # SplitMix64 (Steele, Lea, Flood 2014) from state 0: its first outputs.
+    expected = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC,
+                0x1B39896A51A8749B]
+    x = 0
+    for e in expected:
+        x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        assert OG.mix(x) == e
+
+
+def test_fixed_exp_and_log_track_libm():
+    for v in np.linspace(-40, 0, 4001):
+        assert abs(OG.exp(v) - math.exp(v)) <= 3e-16 * math.exp(v)
+    for v in np.exp(np.linspace(-36.7, 0, 4001)):
+        assert abs(OG.log(v) - math.log(v)) <= 4.5e-16 * max(1.0, abs(math.log(v)))
+
+
+def test_ziggurat_layers_have_equal_area():
+    x = OG.zig_edges()
+    f = lambda t: math.exp(-0.5 * t * t)
+    v = 4.92867323399e-3
+    r = 3.6541528853610088
+    tail = math.sqrt(math.pi / 2) * math.erfc(r / math.sqrt(2))
+    assert abs(r * f(r) + tail - v) < 1e-11
+    assert abs(x[0] * f(r) - v) < 1e-15
+    for i in range(1, 255):
+        assert abs(x[i] * (f(x[i + 1]) - f(x[i])) - v) < 1e-12
+    assert 0 < x[255] < x[254]
+
+
+def test_ghost_stream_is_standard_normal_and_keyed_by_column():
+    A = OG.matrix(np.complex128, 100000, 4, 0xDEADBEEF)
+    z = np.concatenate([A.real.ravel(), A.imag.ravel()]) * np.sqrt(2)
+    assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
+    assert stats.kstest(z, "norm").pvalue > 1e-4
+    # Tail draws exist and are beyond r.
+    assert np.abs(z).max() > 3.7
+    # Width independence: any column range regenerates the same values.
+    assert np.array_equal(OG.ghost(np.complex128, 0, 100000, 3, 2, 0xDEADBEEF), A[:, 2:4])
+    assert np.array_equal(OG.ghost(np.complex128, 500, 7, 2, 1, 0xDEADBEEF), A[500:507, 1:2])
+    assert not np.array_equal(OG.matrix(np.complex128, 100, 4, 0xDEADBEEF + 1), A[:100])
+    R = OG.matrix(np.float64, 200000, 1, 7)
+    assert abs(R.var() - 1) < 0.01
