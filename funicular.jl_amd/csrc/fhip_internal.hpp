@@ -25,7 +25,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 // One 16x16x4 matrix-core product per call, D = A*B + C, on a 64-wide wave.
 // Operand layout: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
-// the result holds D[4 * (l / 16) + r][l % 16] in element r.
+// element r of the result holds D[row(l / 16, r)][l % 16]. The row map differs
+// between the two forms (measured on gfx950): the f64 form interleaves the
+// four lane groups, row = l/16 + 4 r; the f32 form gives each lane group four
+// consecutive rows, row = 4 (l/16) + r.
 template <typename T>
 struct Mfma;
 
@@ -36,6 +39,7 @@ struct Mfma<double> {
   __device__ static inline acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int r) { return lk + 4 * r; }
 };
 
 template <>
@@ -45,6 +49,7 @@ struct Mfma<float> {
   __device__ static inline acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
 };
 
 // How the complex product of one K step combines its four real products.

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(GRAM_THREADS) void gram_partial_kernel(
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int il = mi0 + 16 * mb + 4 * lk + r;
+        const int il = mi0 + 16 * mb + Mfma<TC>::row(lk, r);
         const int jl = nj0 + 16 * nb + li;
         const int64_t off = ((int64_t)jl * TILE + il) * NP;
         out[off] = acc_re[mb][nb][r];

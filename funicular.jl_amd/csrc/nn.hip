@@ -142,7 +142,7 @@ __global__ __launch_bounds__(NN_THREADS) void trsm_runn_kernel(TS *__restrict__ 
     for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = row0 + mi0 + 4 * lk + r;
+        const int64_t row = row0 + mi0 + Mfma<T>::row(lk, r);
         const int col = c0 + 16 * nb + li;
         T re, im;
         load_elem<TS, T, CPLX>(Y, row + (int64_t)col * ldy, row < m && col < k, re, im);
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(NN_THREADS) void trsm_runn_kernel(TS *__restrict__ 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int kk = 16 * nb + li;
-        const int row = mi0 + 4 * lk + r;
+        const int row = mi0 + Mfma<T>::row(lk, r);
         As[0][kk * NN_LDA + row] = acc_re[0][nb][r];
         if (CPLX) As[NP - 1][kk * NN_LDA + row] = acc_im[0][nb][r];
       }
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(NN_THREADS) void trsm_runn_kernel(TS *__restrict__ 
     for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = row0 + mi0 + 4 * lk + r;
+        const int64_t row = row0 + mi0 + Mfma<T>::row(lk, r);
         const int col = c0 + 16 * nb + li;
         if (row < m && col < k)
           store_elem<TS, CPLX>(Y, row + (int64_t)col * ldy, (TS)x_re[0][nb][r], (TS)x_im[0][nb][r]);
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(NN_THREADS) void gemm_nn_kernel(T *__restrict__ D, 
   for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int64_t row = row0 + mi0 + 4 * lk + q;
+      const int64_t row = row0 + mi0 + Mfma<T>::row(lk, q);
       const int col = c0 + 16 * nb + li;
       if (row < m && col < r) store_elem<T, CPLX>(D, row + (int64_t)col * ldd, acc_re[0][nb][q], acc_im[0][nb][q]);
     }

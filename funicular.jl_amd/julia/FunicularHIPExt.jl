@@ -1,0 +1,204 @@
+# FunicularHIPExt: Funicular's device contract (src/device_api.jl:12-138) on
+# libfunicular_hip.so, for a Funicular.jl checkout that declares the stub
+#
+#     function hip_backend(args...; kwargs...)
+#         throw(ArgumentError("the HIP backend arrives with FunicularHIP, which has to be loaded first"))
+#     end
+#
+# next to cuda_backend (src/device_api.jl:173-175) and lists this file as a
+# package extension. It imports only the contract names, as the CUDA
+# extension does (ext/FunicularCUDAExt.jl:8-12), and reaches the GPU through
+# the C ABI declared in include/funicular_hip.h. Nothing here is Julia-side
+# GPU code: arrays are device pointers with a shape and a leading dimension,
+# the kernels are the library's.
+#
+# This image has no Julia, so this file is not exercised here; the Python host
+# package funicular_hip is the same mapping and is what the tests run.
+module FunicularHIPExt
+
+import Funicular: alloc_device, alloc_host_slab, d2h!, devicetype, h2d!, hip_backend, ka_backend,
+                  make_queue, make_sweep_queues, record_event, submit!, sync_all, sync_event, sync_queue,
+                  wait_event, supports_eltype, gram_accumulate!, rightmul_gemm!, rdiv_upper!, convert_copy!
+
+using Funicular: DeviceBackend, SLAB_ALIGN, aligned_slab
+using LinearAlgebra
+
+const LIB = Ref{String}(get(ENV, "FUNICULAR_HIP_LIB", joinpath(@__DIR__, "..", "funicular_hip", "libfunicular_hip.so")))
+
+const FHIP_F32, FHIP_F64, FHIP_C32, FHIP_C64 = 0, 1, 2, 3
+const FHIP_H2D, FHIP_D2H, FHIP_D2D = 1, 2, 3
+
+eltcode(::Type{Float32}) = FHIP_F32
+eltcode(::Type{Float64}) = FHIP_F64
+eltcode(::Type{ComplexF32}) = FHIP_C32
+eltcode(::Type{ComplexF64}) = FHIP_C64
+
+function check(status::Cint)
+    status == 0 && return nothing
+    msg = unsafe_string(ccall((:fhip_last_error, LIB[]), Cstring, ()))
+    status == 2 && throw(DimensionMismatch(msg))
+    status == 1 && throw(ArgumentError(msg))
+    error(msg)
+end
+
+struct HIPBackend <: DeviceBackend end
+hip_backend() = HIPBackend()
+
+# A device matrix: pointer, shape, leading dimension. Views keep the parent's
+# leading dimension (the CUDA extension's region_pointer/copy_shape,
+# ext/FunicularCUDAExt.jl:157-182).
+mutable struct HIPMatrix{T} <: AbstractMatrix{T}
+    ptr::Ptr{T}
+    dims::Dims{2}
+    ld::Int
+    owner::Any
+end
+Base.size(A::HIPMatrix) = A.dims
+Base.getindex(::HIPMatrix, ::Int...) = throw(ArgumentError("HIPMatrix has no scalar indexing; copy it to the host"))
+Base.view(A::HIPMatrix{T}, rows::AbstractUnitRange, cols::AbstractUnitRange) where {T} =
+    HIPMatrix{T}(A.ptr + ((first(rows) - 1) + (first(cols) - 1) * A.ld) * sizeof(T), (length(rows), length(cols)), A.ld, A.owner)
+Base.view(A::HIPMatrix, ::Colon, cols::AbstractUnitRange) = view(A, 1:size(A, 1), cols)
+Base.view(A::HIPMatrix, rows::AbstractUnitRange, ::Colon) = view(A, rows, 1:size(A, 2))
+
+devicetype(::HIPBackend, ::Type{T}) where {T} = HIPMatrix{T}
+supports_eltype(::HIPBackend, ::Type{T}) where {T} = T in (Float32, Float64, ComplexF32, ComplexF64)
+ka_backend(::HIPBackend) = nothing  # no KernelAbstractions kernels run on this backend
+
+function alloc_device(::HIPBackend, ::Type{T}, dims::Dims) where {T}
+    d2 = length(dims) == 1 ? (dims[1], 1) : dims
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fhip_malloc, LIB[]), Cint, (Ptr{Ptr{Cvoid}}, Csize_t), p, max(prod(d2) * sizeof(T), 1)))
+    owner = p[]
+    A = HIPMatrix{T}(Ptr{T}(owner), d2, max(d2[1], 1), owner)
+    finalizer(a -> ccall((:fhip_free, LIB[]), Cint, (Ptr{Cvoid},), a.owner), A)
+    A
+end
+
+function alloc_host_slab(::HIPBackend, nbytes::Integer)
+    p = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fhip_host_alloc, LIB[]), Cint, (Ptr{Ptr{Cvoid}}, Csize_t), p, nbytes + SLAB_ALIGN))
+    owner = Ref(p[])
+    finalizer(o -> ccall((:fhip_host_free, LIB[]), Cint, (Ptr{Cvoid},), o[]), owner)
+    aligned_slab(owner, Ptr{UInt8}(p[]), nbytes)
+end
+
+struct HIPQueue
+    stream::Ptr{Cvoid}
+end
+
+function make_queue(::HIPBackend)
+    s = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fhip_stream_create, LIB[]), Cint, (Ptr{Ptr{Cvoid}},), s))
+    HIPQueue(s[])
+end
+
+mutable struct HIPEvent
+    handle::Ptr{Cvoid}
+end
+
+# The stream that kernels launched from a panel function land on: the task
+# that runs submit! stores it here (the task-local stream CUDA.jl gave the
+# CUDA extension, ext/FunicularCUDAExt.jl:83-89).
+current_stream() = get(task_local_storage(), :fhip_stream, C_NULL)
+
+function record_event(::HIPBackend, q::HIPQueue)
+    e = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fhip_event_create, LIB[]), Cint, (Ptr{Ptr{Cvoid}}, Cint), e, 1))
+    ev = HIPEvent(e[])
+    finalizer(x -> ccall((:fhip_event_destroy, LIB[]), Cint, (Ptr{Cvoid},), x.handle), ev)
+    check(ccall((:fhip_event_record, LIB[]), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), ev.handle, q.stream))
+    ev
+end
+
+wait_event(::HIPBackend, q::HIPQueue, e::HIPEvent) =
+    (check(ccall((:fhip_stream_wait_event, LIB[]), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), q.stream, e.handle)); nothing)
+sync_queue(::HIPBackend, q::HIPQueue) = (check(ccall((:fhip_stream_sync, LIB[]), Cint, (Ptr{Cvoid},), q.stream)); nothing)
+sync_event(::HIPBackend, e::HIPEvent) = (check(ccall((:fhip_event_sync, LIB[]), Cint, (Ptr{Cvoid},), e.handle)); nothing)
+sync_all(::HIPBackend) = (check(ccall((:fhip_device_sync, LIB[]), Cint, ())); nothing)
+
+function submit!(b::HIPBackend, q::HIPQueue, work)
+    task_local_storage(work, :fhip_stream, q.stream)
+    record_event(b, q)
+end
+
+# Every copy is one 2-D copy with both pitches (ext/FunicularCUDAExt.jl:134-152).
+region(A::HIPMatrix{T}) where {T} = (Ptr{Cvoid}(A.ptr), A.ld * sizeof(T))
+function region(A::StridedMatrix{T}) where {T}
+    stride(A, 1) == 1 || throw(ArgumentError("a copy needs its columns contiguous"))
+    (Ptr{Cvoid}(pointer(A)), stride(A, 2) * sizeof(T))
+end
+
+function copy2d!(dst, src, kind, queue)
+    size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    isempty(dst) && return nothing
+    T = eltype(dst)
+    dp, dpitch = region(dst)
+    sp, spitch = region(src)
+    stream = queue === nothing ? C_NULL : queue.stream
+    GC.@preserve dst src check(ccall((:fhip_memcpy2d_async, LIB[]), Cint,
+        (Ptr{Cvoid}, Csize_t, Ptr{Cvoid}, Csize_t, Csize_t, Csize_t, Cint, Ptr{Cvoid}),
+        dp, dpitch, sp, spitch, size(dst, 1) * sizeof(T), size(dst, 2), kind, stream))
+    queue === nothing && check(ccall((:fhip_stream_sync, LIB[]), Cint, (Ptr{Cvoid},), C_NULL))
+    nothing
+end
+
+h2d!(dst::HIPMatrix, src, b::HIPBackend; queue=nothing) =
+    (copy2d!(dst, src, FHIP_H2D, queue); queue === nothing ? nothing : record_event(b, queue))
+d2h!(dst, src::HIPMatrix, b::HIPBackend; queue=nothing) =
+    (copy2d!(dst, src, FHIP_D2H, queue); queue === nothing ? nothing : record_event(b, queue))
+
+# The kernel overrides (the Metal extension's pattern, ext/FunicularMetalExt.jl:187-189).
+function gram_accumulate!(::HIPBackend, C::HIPMatrix{T}, A::HIPMatrix{T}, B::HIPMatrix{T}, α, β) where {T}
+    herm = Cint(A.ptr == B.ptr && A.ld == B.ld && size(A, 2) == size(B, 2))
+    n = Ref{Csize_t}(0)
+    check(ccall((:fhip_gram_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Int64, Int64, Cint, Cint, Cint),
+                n, size(A, 1), size(A, 2), size(B, 2), eltcode(T), eltcode(T), herm))
+    work = workspace(n[])
+    a, bb = ComplexF64(α), ComplexF64(β)
+    check(ccall((:fhip_gram, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Int64, Float64, Float64, Float64, Float64,
+         Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+        C.ptr, C.ld, A.ptr, A.ld, B.ptr, B.ld, size(A, 1), size(A, 2), size(B, 2), real(a), imag(a), real(bb), imag(bb),
+        eltcode(T), eltcode(T), work.ptr, n[], current_stream()))
+    C
+end
+
+function rdiv_upper!(::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}) where {T}
+    n = Ref{Csize_t}(0)
+    check(ccall((:fhip_trsm_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Cint), n, size(R, 1), eltcode(T)))
+    work = workspace(n[])
+    check(ccall((:fhip_trsm_runn, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+        Y.ptr, Y.ld, size(Y, 1), size(Y, 2), R.ptr, R.ld, eltcode(T), eltcode(T), work.ptr, n[], current_stream()))
+    Y
+end
+
+function rightmul_gemm!(::HIPBackend, dst::HIPMatrix{T}, A::HIPMatrix{T}, C::HIPMatrix{T}) where {T}
+    check(ccall((:fhip_gemm_nn, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Int64, Cint, Ptr{Cvoid}),
+        dst.ptr, dst.ld, A.ptr, A.ld, C.ptr, C.ld, size(A, 1), size(A, 2), size(C, 2), eltcode(T), current_stream()))
+    dst
+end
+
+function convert_copy!(::HIPBackend, dst::HIPMatrix{S}, src::HIPMatrix{T}) where {S,T}
+    size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    check(ccall((:fhip_convert_copy, LIB[]), Cint,
+        (Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Cint, Int64, Int64, Int64, Ptr{Cvoid}),
+        dst.ptr, eltcode(S), dst.ld, src.ptr, eltcode(T), src.ld, size(dst, 1), size(dst, 2), current_stream()))
+    dst
+end
+
+# A per-backend workspace that grows to the largest request and is reused;
+# the first sweep of a shape allocates it, later ones do not.
+const WORK = Ref{Any}(nothing)
+function workspace(n)
+    w = WORK[]
+    if w === nothing || w.nbytes < n
+        w = alloc_device(HIPBackend(), UInt8, (max(Int(n), 1), 1))
+        WORK[] = (ptr=w.ptr, nbytes=Int(n), owner=w)
+        w = WORK[]
+    end
+    w
+end
+
+end # module FunicularHIPExt
