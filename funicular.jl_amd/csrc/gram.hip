@@ -16,6 +16,8 @@
 //    the bits depend on (m, ka, kb) only.
 #include "fhip_internal.hpp"
 
+#include <cmath>
+
 namespace fhip {
 
 template <int TILE>
@@ -32,7 +34,6 @@ struct GramShape<32> {
 };
 
 constexpr int GRAM_THREADS = 256;
-constexpr int GRAM_TARGET_WGS = 4096;
 constexpr int GRAM_MIN_CHUNK_ROWS = 256;
 
 // Upper-triangle tile t -> (ti, tj), tiles numbered column by column.
@@ -52,6 +53,14 @@ struct GramPlan {
   int nslots;         // 8 * q * ksplit
 };
 
+// Workgroups the MI355X keeps resident at once for the partial kernel: 256
+// CUs x 4 (LDS-limited). The chunk count is chosen so the grid fills whole
+// waves of that many workgroups, which removes the half-empty last wave. It
+// is a constant, not a device query, so the chunking -- and with it every
+// bit of the result -- depends on (m, ka, kb) alone.
+constexpr int GRAM_RESIDENT_WGS = 1024;
+constexpr int GRAM_MAX_Q = 256;
+
 static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm) {
   GramPlan p;
   p.tile = (ka > 32 || kb > 32) ? 64 : 32;
@@ -60,11 +69,23 @@ static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm) {
   p.ntm = (int)cdiv64(ka, p.tile);
   p.ntn = (int)cdiv64(kb, p.tile);
   p.ntiles = herm ? p.ntm * (p.ntm + 1) / 2 : p.ntm * p.ntn;
-  int64_t want = cdiv64(GRAM_TARGET_WGS, p.ntiles);
-  int64_t by_rows = cdiv64(m > 0 ? m : 1, GRAM_MIN_CHUNK_ROWS);
-  int64_t chunks = want < by_rows ? want : by_rows;
-  if (chunks < 1) chunks = 1;
-  p.q = (int)cdiv64(chunks, 8);
+  // At most one chunk per GRAM_MIN_CHUNK_ROWS rows.
+  int64_t qmax = cdiv64(cdiv64(m > 0 ? m : 1, GRAM_MIN_CHUNK_ROWS), 8);
+  if (qmax > GRAM_MAX_Q) qmax = GRAM_MAX_Q;
+  int best_q = 1;
+  double best = -1.0;
+  for (int q = 1; q <= qmax; ++q) {
+    const double n = 8.0 * q * p.ntiles;
+    const double waves = std::ceil(n / GRAM_RESIDENT_WGS);
+    double score = n / (waves * GRAM_RESIDENT_WGS);
+    if (n < 2.0 * GRAM_RESIDENT_WGS) score *= n / (2.0 * GRAM_RESIDENT_WGS);
+    if (score > best + 1e-9) {
+      best = score;
+      best_q = q;
+    }
+    if (n >= 16.0 * GRAM_RESIDENT_WGS) break;
+  }
+  p.q = best_q;
   int64_t rows = cdiv64(m > 0 ? m : 1, 8 * (int64_t)p.q);
   p.chunk_rows = cdiv64(rows, kc) * kc;
   p.nslots = 8 * p.q * p.ksplit;

@@ -117,79 +117,146 @@ __global__ __launch_bounds__(NN_CB) void trsm_diag_inv_kernel(const TS *__restri
   }
 }
 
+// The solve proper. A workgroup owns TR_RW rows; wave w holds rows
+// [32 w, 32 w + 32) x the 32 columns of the current block as 2 x 2 blocks of
+// the 16x16 matrix-core tile. The update streams TR_KC columns of the solved
+// rows and of R at a time, the next slice's loads in flight during the
+// current slice's products.
+constexpr int TR_RW = 128;
+constexpr int TR_KC = 16;
+constexpr int TR_LDA = TR_RW + 4;
+constexpr int TR_LDB = NN_CB + 4;
+constexpr int TR_XPT = TR_KC * TR_RW / NN_THREADS;  // X elements per thread per slice (8)
+constexpr int TR_RPT = TR_KC * NN_CB / NN_THREADS;  // R elements per thread per slice (2)
+
 template <typename TS, typename T, bool CPLX>
-__global__ __launch_bounds__(NN_THREADS) void trsm_runn_kernel(TS *__restrict__ Y, int64_t ldy, int64_t m,
+__global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void trsm_runn_kernel(TS *__restrict__ Y, int64_t ldy, int64_t m,
                                                                 int k, const TS *__restrict__ R, int64_t ldr,
                                                                 const T *__restrict__ Dinv) {
   constexpr int NP = CPLX ? 2 : 1;
-  __shared__ T As[NP][NN_KC * NN_LDA];
-  __shared__ T Bs[NP][NN_KC * NN_LDB];
+  __shared__ T As[NP][TR_KC * TR_LDA];
+  __shared__ T Bs[NP][TR_KC * TR_LDB];
   typedef typename Mfma<T>::acc acc_t;
 
-  const int64_t row0 = (int64_t)blockIdx.x * NN_RW;
+  const int64_t row0 = (int64_t)blockIdx.x * TR_RW;
   const int t = threadIdx.x;
   const int w = t >> 6;
   const int lane = t & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
-  const int mi0 = 16 * w;  // the wave's 16 rows
+  const int mi0 = 32 * w;
   const int nbk = (k + NN_CB - 1) / NN_CB;
+
+  T xr[TR_XPT], xi[TR_XPT], rr[TR_RPT], ri[TR_RPT];
+  // Slice kc of the update: X[row0 + r, kc + kk] and R[kc + kk, c0 + c].
+  auto fetch = [&](int kc, int c0) {
+#pragma unroll
+    for (int e = 0; e < TR_XPT; ++e) {
+      const int idx = t + NN_THREADS * e;
+      const int r = idx % TR_RW;
+      const int kk = idx / TR_RW;
+      const int64_t row = row0 + r;
+      load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, row < m, xr[e], xi[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < TR_RPT; ++e) {
+      const int idx = t + NN_THREADS * e;
+      const int kk = idx % TR_KC;
+      const int c = idx / TR_KC;
+      load_elem<TS, T, CPLX>(R, (int64_t)(kc + kk) + (int64_t)(c0 + c) * ldr, c0 + c < k, rr[e], ri[e]);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int e = 0; e < TR_XPT; ++e) {
+      const int idx = t + NN_THREADS * e;
+      const int r = idx % TR_RW;
+      const int kk = idx / TR_RW;
+      As[0][kk * TR_LDA + r] = xr[e];
+      if (CPLX) As[NP - 1][kk * TR_LDA + r] = xi[e];
+    }
+#pragma unroll
+    for (int e = 0; e < TR_RPT; ++e) {
+      const int idx = t + NN_THREADS * e;
+      const int kk = idx % TR_KC;
+      const int c = idx / TR_KC;
+      Bs[0][kk * TR_LDB + c] = rr[e];
+      if (CPLX) Bs[NP - 1][kk * TR_LDB + c] = ri[e];
+    }
+  };
 
   for (int J = 0; J < nbk; ++J) {
     const int c0 = J * NN_CB;
-    acc_t acc_re[1][2], acc_im[1][2];
+    acc_t acc_re[2][2], acc_im[2][2];
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = row0 + mi0 + Mfma<T>::row(lk, r);
-        const int col = c0 + 16 * nb + li;
-        T re, im;
-        load_elem<TS, T, CPLX>(Y, row + (int64_t)col * ldy, row < m && col < k, re, im);
-        acc_re[0][nb][r] = re;
-        acc_im[0][nb][r] = im;
-      }
-    }
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
+          const int col = c0 + 16 * nb + li;
+          T re, im;
+          load_elem<TS, T, CPLX>(Y, row + (int64_t)col * ldy, row < m && col < k, re, im);
+          acc_re[mb][nb][r] = re;
+          acc_im[mb][nb][r] = im;
+        }
     // Trailing update from the blocks already solved.
-    for (int kc = 0; kc < c0; kc += NN_KC) {
+    if (c0 > 0) fetch(0, c0);
+    for (int kc = 0; kc < c0; kc += TR_KC) {
       __syncthreads();
-      stage_rows<TS, T, CPLX>(As[0], As[NP - 1], Y, ldy, row0, m, kc, c0);
-      stage_cols<TS, T, CPLX>(Bs[0], Bs[NP - 1], R, ldr, kc, c0, c0, k);
+      stash();
       __syncthreads();
-      mma_steps<T, CPLX, 1, 2, PROD_SUB>(As[0], As[NP - 1], NN_LDA, mi0, Bs[0], Bs[NP - 1], NN_LDB, 0, 0,
-                                         NN_KC / 4, 1, acc_re, acc_im);
+      if (kc + TR_KC < c0) fetch(kc + TR_KC, c0);
+      mma_steps<T, CPLX, 2, 2, PROD_SUB>(As[0], As[NP - 1], TR_LDA, mi0, Bs[0], Bs[NP - 1], TR_LDB, 0, 0,
+                                         TR_KC / 4, 1, acc_re, acc_im);
     }
-    // Multiply by the inverse of the diagonal block.
-    __syncthreads();
+    // Multiply by the inverse of the diagonal block, one 16-column half at a time.
+    acc_t x_re[2][2], x_im[2][2];
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int kk = 16 * nb + li;
-        const int row = mi0 + Mfma<T>::row(lk, r);
-        As[0][kk * NN_LDA + row] = acc_re[0][nb][r];
-        if (CPLX) As[NP - 1][kk * NN_LDA + row] = acc_im[0][nb][r];
+      for (int nb = 0; nb < 2; ++nb) {
+        x_re[mb][nb] = Mfma<T>::zero();
+        x_im[mb][nb] = Mfma<T>::zero();
       }
-    stage_cols<T, T, CPLX>(Bs[0], Bs[NP - 1], Dinv + (int64_t)J * NN_CB * NN_CB * NP, NN_CB, 0, NN_CB, 0,
-                           NN_CB);
-    __syncthreads();
-    acc_t x_re[1][2], x_im[1][2];
+    const T *Dj = Dinv + (int64_t)J * NN_CB * NN_CB * NP;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      x_re[0][nb] = Mfma<T>::zero();
-      x_im[0][nb] = Mfma<T>::zero();
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mi0 + 16 * mb + Mfma<T>::row(lk, r);
+          As[0][li * TR_LDA + row] = acc_re[mb][h][r];
+          if (CPLX) As[NP - 1][li * TR_LDA + row] = acc_im[mb][h][r];
+        }
+#pragma unroll
+      for (int e = 0; e < TR_RPT; ++e) {
+        const int idx = t + NN_THREADS * e;
+        const int kk = idx % TR_KC;
+        const int c = idx / TR_KC;
+        T re, im;
+        load_elem<T, T, CPLX>(Dj, (int64_t)(16 * h + kk) + (int64_t)c * NN_CB, true, re, im);
+        Bs[0][kk * TR_LDB + c] = re;
+        if (CPLX) Bs[NP - 1][kk * TR_LDB + c] = im;
+      }
+      __syncthreads();
+      mma_steps<T, CPLX, 2, 2, PROD_ADD>(As[0], As[NP - 1], TR_LDA, mi0, Bs[0], Bs[NP - 1], TR_LDB, 0, 0,
+                                         TR_KC / 4, 1, x_re, x_im);
     }
-    mma_steps<T, CPLX, 1, 2, PROD_ADD>(As[0], As[NP - 1], NN_LDA, mi0, Bs[0], Bs[NP - 1], NN_LDB, 0, 0,
-                                       NN_CB / 4, 1, x_re, x_im);
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
+    for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t row = row0 + mi0 + Mfma<T>::row(lk, r);
-        const int col = c0 + 16 * nb + li;
-        if (row < m && col < k)
-          store_elem<TS, CPLX>(Y, row + (int64_t)col * ldy, (TS)x_re[0][nb][r], (TS)x_im[0][nb][r]);
-      }
+      for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
+          const int col = c0 + 16 * nb + li;
+          if (row < m && col < k)
+            store_elem<TS, CPLX>(Y, row + (int64_t)col * ldy, (TS)x_re[mb][nb][r], (TS)x_im[mb][nb][r]);
+        }
     // The stores above are read back by the next block's update: the barrier
     // is a workgroup-scope release/acquire.
     __syncthreads();
@@ -255,7 +322,7 @@ static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_
   hipLaunchKernelGGL((trsm_diag_inv_kernel<TS, T, CPLX>), dim3(nbk), dim3(NN_CB), 0, stream, R, ldr, k, Dinv);
   int st = check_launch("trsm_diag_inv_kernel");
   if (st || m == 0) return st;
-  const int64_t grid = cdiv64(m, NN_RW);
+  const int64_t grid = cdiv64(m, TR_RW);
   hipLaunchKernelGGL((trsm_runn_kernel<TS, T, CPLX>), dim3((unsigned)grid), dim3(NN_THREADS), 0, stream, Y, ldy,
                      m, k, R, ldr, (const T *)Dinv);
   return check_launch("trsm_runn_kernel");
