@@ -140,9 +140,56 @@ __global__ __launch_bounds__(256) void helmholtz_kernel(T *__restrict__ Y, int64
   }
 }
 
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void axpby_kernel(T *__restrict__ Y, int64_t ldy, const T *__restrict__ X,
+                                                    int64_t ldx, int64_t rows, int64_t cols, T ar, T ai, T br,
+                                                    T bi) {
+  const int64_t n = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % rows;
+    const int64_t j = e / rows;
+    T xr, xi, yr, yi;
+    load_elem<T, T, C>(X, i + j * ldx, true, xr, xi);
+    load_elem<T, T, C>(Y, i + j * ldy, true, yr, yi);
+    const T outr = ar * xr - ai * xi + br * yr - bi * yi;
+    const T outi = ar * xi + ai * xr + br * yi + bi * yr;
+    store_elem<T, C>(Y, i + j * ldy, outr, outi);
+  }
+}
+
 }  // namespace fhip
 
 using namespace fhip;
+
+extern "C" int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t rows, int64_t cols,
+                          double a_re, double a_im, double b_re, double b_im, int type, void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_axpby: unknown eltype");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_axpby: negative dimension");
+  if (rows == 0 || cols == 0) return FHIP_OK;
+  if (!Y || !X) return fail(FHIP_E_ARGUMENT, "fhip_axpby: null pointer");
+  if (ldy < rows || ldx < rows) return fail(FHIP_E_ARGUMENT, "fhip_axpby: leading dimension smaller than the rows");
+  const unsigned g = grid_for(rows * cols);
+  hipStream_t s = (hipStream_t)stream;
+  switch (type) {
+    case FHIP_F32:
+      hipLaunchKernelGGL((axpby_kernel<float, false>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X,
+                         ldx, rows, cols, (float)a_re, 0.0f, (float)b_re, 0.0f);
+      break;
+    case FHIP_F64:
+      hipLaunchKernelGGL((axpby_kernel<double, false>), dim3(g), dim3(256), 0, s, (double *)Y, ldy,
+                         (const double *)X, ldx, rows, cols, a_re, 0.0, b_re, 0.0);
+      break;
+    case FHIP_C32:
+      hipLaunchKernelGGL((axpby_kernel<float, true>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X,
+                         ldx, rows, cols, (float)a_re, (float)a_im, (float)b_re, (float)b_im);
+      break;
+    case FHIP_C64:
+      hipLaunchKernelGGL((axpby_kernel<double, true>), dim3(g), dim3(256), 0, s, (double *)Y, ldy,
+                         (const double *)X, ldx, rows, cols, a_re, a_im, b_re, b_im);
+      break;
+  }
+  return check_launch("axpby_kernel");
+}
 
 extern "C" int fhip_convert_copy(void *dst, int dst_type, int64_t ldd, const void *src, int src_type,
                                  int64_t lds, int64_t rows, int64_t cols, void *stream) {

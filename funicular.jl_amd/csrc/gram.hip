@@ -22,15 +22,30 @@ namespace fhip {
 
 template <int TILE>
 struct GramShape;
+// KC rows are staged per LDS pass; KSPLIT waves split the K steps of one tile
+// (the 32 tile), otherwise the 4 waves take the four quadrants of the tile;
+// MB x NB is a wave's block count in 16x16 matrix-core tiles; RESIDENT is how
+// many such workgroups the MI355X holds at once (256 CUs x LDS/register limit).
+template <>
+struct GramShape<128> {
+  static constexpr int KC = 16;
+  static constexpr int KSPLIT = 1;
+  static constexpr int MB = 4, NB = 4;
+  static constexpr int RESIDENT = 512;
+};
 template <>
 struct GramShape<64> {
-  static constexpr int KC = 16;     // rows staged per LDS pass
-  static constexpr int KSPLIT = 1;  // waves that split the K steps of one tile
+  static constexpr int KC = 16;
+  static constexpr int KSPLIT = 1;
+  static constexpr int MB = 2, NB = 2;
+  static constexpr int RESIDENT = 1024;
 };
 template <>
 struct GramShape<32> {
   static constexpr int KC = 32;
   static constexpr int KSPLIT = 4;
+  static constexpr int MB = 2, NB = 2;
+  static constexpr int RESIDENT = 1024;
 };
 
 constexpr int GRAM_THREADS = 256;
@@ -53,19 +68,26 @@ struct GramPlan {
   int nslots;         // 8 * q * ksplit
 };
 
-// Workgroups the MI355X keeps resident at once for the partial kernel: 256
-// CUs x 4 (LDS-limited). The chunk count is chosen so the grid fills whole
-// waves of that many workgroups, which removes the half-empty last wave. It
-// is a constant, not a device query, so the chunking -- and with it every
-// bit of the result -- depends on (m, ka, kb) alone.
-constexpr int GRAM_RESIDENT_WGS = 1024;
+// The chunk count is chosen so the grid fills whole waves of the resident
+// workgroup count of the tile shape, which removes a half-empty last wave. It
+// is a constant of the shape, not a device query, so the chunking -- and with
+// it every bit of the result -- depends on (m, ka, kb, compute eltype) alone.
 constexpr int GRAM_MAX_Q = 256;
 
-static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm) {
+static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm, bool fp32) {
   GramPlan p;
-  p.tile = (ka > 32 || kb > 32) ? 64 : 32;
-  const int kc = p.tile == 64 ? GramShape<64>::KC : GramShape<32>::KC;
-  p.ksplit = p.tile == 64 ? GramShape<64>::KSPLIT : GramShape<32>::KSPLIT;
+  // FP32 products run at twice the FP64 rate, so the single-precision Gram
+  // takes 128-wide tiles to halve the operand traffic per flop.
+  p.tile = (ka > 64 && kb > 64 && fp32) ? 128 : (ka > 32 || kb > 32) ? 64 : 32;
+  int kc, resident;
+  if (p.tile == 128) {
+    kc = GramShape<128>::KC; p.ksplit = GramShape<128>::KSPLIT; resident = GramShape<128>::RESIDENT;
+  } else if (p.tile == 64) {
+    kc = GramShape<64>::KC; p.ksplit = GramShape<64>::KSPLIT; resident = GramShape<64>::RESIDENT;
+  } else {
+    kc = GramShape<32>::KC; p.ksplit = GramShape<32>::KSPLIT; resident = GramShape<32>::RESIDENT;
+  }
+  const int GRAM_RESIDENT_WGS = resident;
   p.ntm = (int)cdiv64(ka, p.tile);
   p.ntn = (int)cdiv64(kb, p.tile);
   p.ntiles = herm ? p.ntm * (p.ntm + 1) / 2 : p.ntm * p.ntn;
@@ -93,14 +115,16 @@ static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm) {
 }
 
 template <typename TIn, typename TC, bool CPLX, int TILE>
-__global__ __launch_bounds__(GRAM_THREADS) void gram_partial_kernel(
+__global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void gram_partial_kernel(
     const TIn *__restrict__ A, int64_t lda, const TIn *__restrict__ B, int64_t ldb, int64_t m, int ka,
     int kb, int64_t chunk_rows, int ntiles, int ntm, int herm, TC *__restrict__ partial) {
   constexpr int KC = GramShape<TILE>::KC;
   constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
+  constexpr int MB = GramShape<TILE>::MB;
+  constexpr int NB = GramShape<TILE>::NB;
   constexpr int LDS_W = TILE + 4;
   constexpr int NP = CPLX ? 2 : 1;
-  constexpr int PER_THREAD = KC * TILE / GRAM_THREADS;  // 4 in both shapes
+  constexpr int PER_THREAD = KC * TILE / GRAM_THREADS;
   __shared__ TC As[NP][KC * LDS_W];
   __shared__ TC Bs[NP][KC * LDS_W];
 
@@ -127,18 +151,18 @@ __global__ __launch_bounds__(GRAM_THREADS) void gram_partial_kernel(
   const int lane = t & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
-  // The wave's 32x32 sub-tile and the K steps it owns.
-  const int mi0 = (TILE == 64) ? 32 * (w & 1) : 0;
-  const int nj0 = (TILE == 64) ? 32 * (w >> 1) : 0;
+  // The wave's sub-tile (a quadrant, or the whole 32 tile) and the K steps it owns.
+  const int mi0 = (KSPLIT == 1) ? (TILE / 2) * (w & 1) : 0;
+  const int nj0 = (KSPLIT == 1) ? (TILE / 2) * (w >> 1) : 0;
   const int s_begin = (KSPLIT > 1) ? w : 0;
   const int s_stride = KSPLIT;
 
   typedef typename Mfma<TC>::acc acc_t;
-  acc_t acc_re[2][2], acc_im[2][2];
+  acc_t acc_re[MB][NB], acc_im[MB][NB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < NB; ++b) {
       acc_re[a][b] = Mfma<TC>::zero();
       acc_im[a][b] = Mfma<TC>::zero();
     }
@@ -179,7 +203,7 @@ __global__ __launch_bounds__(GRAM_THREADS) void gram_partial_kernel(
     if (rb + KC < r1) fetch(rb + KC);
     const TC *bre = same ? As[0] : Bs[0];
     const TC *bim = same ? As[NP - 1] : Bs[NP - 1];
-    mma_steps<TC, CPLX, 2, 2, PROD_CONJ_A>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
+    mma_steps<TC, CPLX, MB, NB, PROD_CONJ_A>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
                                            KC / 4, s_stride, acc_re, acc_im);
   }
 
@@ -187,9 +211,9 @@ __global__ __launch_bounds__(GRAM_THREADS) void gram_partial_kernel(
   const int slot = chunk * KSPLIT + ((KSPLIT > 1) ? w : 0);
   TC *out = partial + ((int64_t)slot * ntiles + tile) * (TILE * TILE) * NP;
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
+  for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb)
+    for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int il = mi0 + 16 * mb + Mfma<TC>::row(lk, r);
@@ -280,6 +304,11 @@ template <typename TIn, typename TC, typename TOut, bool CPLX>
 static int dispatch_gram(const GramPlan &p, void *C, int64_t ldc, const void *A, int64_t lda,
                          const void *B, int64_t ldb, int64_t m, int ka, int kb, bool herm, double ar,
                          double ai, double br, double bi, void *work, hipStream_t stream) {
+  if constexpr (sizeof(TC) == 4) {
+    if (p.tile == 128)
+      return launch_gram<TIn, TC, TOut, CPLX, 128>(p, (TOut *)C, ldc, (const TIn *)A, lda, (const TIn *)B,
+                                                   ldb, m, ka, kb, herm, ar, ai, br, bi, work, stream);
+  }
   if (p.tile == 64)
     return launch_gram<TIn, TC, TOut, CPLX, 64>(p, (TOut *)C, ldc, (const TIn *)A, lda, (const TIn *)B,
                                                 ldb, m, ka, kb, herm, ar, ai, br, bi, work, stream);
@@ -314,7 +343,7 @@ extern "C" int fhip_gram_workspace_bytes(size_t *nbytes, int64_t m, int64_t ka, 
     *nbytes = 0;
     return FHIP_OK;
   }
-  GramPlan p = gram_plan(m, ka, kb, hermitian != 0 && ka == kb);
+  GramPlan p = gram_plan(m, ka, kb, hermitian != 0 && ka == kb, !is_double(compute));
   *nbytes = gram_work_bytes(p, compute);
   return FHIP_OK;
 }
@@ -334,7 +363,7 @@ extern "C" int fhip_gram(void *C, int64_t ldc, const void *A, int64_t lda, const
     return fail(FHIP_E_ARGUMENT, "fhip_gram: leading dimension smaller than the matrix (ldc=%lld lda=%lld ldb=%lld)",
                 (long long)ldc, (long long)lda, (long long)ldb);
   const bool herm = (A == B) && (lda == ldb) && (ka == kb);
-  GramPlan p = gram_plan(m, ka, kb, herm);
+  GramPlan p = gram_plan(m, ka, kb, herm, !is_double(compute));
   const size_t need = gram_work_bytes(p, compute);
   if (!work || work_bytes < need)
     return fail(FHIP_E_WORKSPACE, "fhip_gram: workspace of %zu bytes is smaller than the %zu needed",

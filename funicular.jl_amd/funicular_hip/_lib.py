@@ -84,6 +84,7 @@ _SIGNATURES = {
     "fhip_gemm_nn": [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i, _vp],
     "fhip_convert_copy": [_vp, _i, _i64, _vp, _i, _i64, _i64, _i64, _vp],
     "fhip_fill": [_vp, _i, _i64, _i64, _i64, _d, _d, _vp],
+    "fhip_axpby": [_vp, _i64, _vp, _i64, _i64, _i64, _d, _d, _d, _d, _i, _vp],
     "fhip_ghost_randn": [_vp, _i, _i64, _i64, _i64, _i64, _i64, _u64, _vp],
     "fhip_ghost_tables_checksum": [ctypes.POINTER(_u64)],
     "fhip_stencil_tridiag": [_vp, _i64, _vp, _i64, _i64, _i64, _d, _d, _d, _d, _vp, _i, _i, _vp],

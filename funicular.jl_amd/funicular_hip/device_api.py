@@ -524,6 +524,15 @@ class HIPBackend(DeviceBackend):
              src.ld, dst.rows, dst.cols, s)
         return dst
 
+    def axpby(self, Y, X, a, b):
+        """Y <- a X + b Y."""
+        if tuple(Y.shape) != tuple(X.shape) or Y.dtype != X.dtype:
+            raise DimensionMismatch(f"axpby between a {Y.shape} {Y.dtype} and a {X.shape} {X.dtype} matrix")
+        a, b = complex(a), complex(b)
+        call("fhip_axpby", Y.ptr, Y.ld, X.ptr, X.ld, Y.rows, Y.cols, a.real, a.imag, b.real, b.imag,
+             eltype_code(Y.dtype), current_stream())
+        return Y
+
     def fill(self, dst, value):
         v = complex(value)
         call("fhip_fill", dst.ptr, eltype_code(dst.dtype), dst.ld, dst.rows, dst.cols, v.real, v.imag,

@@ -260,42 +260,56 @@ def project(Q, G, *, nbuffers=None):
 UNIT_ROUNDOFF_FP32 = 2.0 ** -24
 
 
-def cholesky_upper_shifted_fp32(G, N, k):
-    """The first pass's preconditioning factor: Cholesky in FP32 of
-    G + s I with s = 11 (N k + k (k + 1)) u32 ||Y||_2^2, ||Y||_2^2 the largest
-    eigenvalue of the Gram (shifted CholeskyQR, Fukaya et al. 2020). This has
-    no counterpart in the reference, whose shift rule (src/panelblas.jl:151)
-    is for an FP64 Gram that failed outright."""
+def cholesky_upper_fp32(G, N, k, shift="adaptive"):
+    """The first pass's preconditioning factor, a Cholesky factorization in
+    FP32 of the Gram formed on the FP32 matrix cores. Returns (R, s) with s
+    the diagonal shift that was applied.
+
+    shift="fukaya" always factors G + s I with s = 11 (N k + k (k+1)) u32
+    ||Y||_2^2 (shifted CholeskyQR, Fukaya et al. 2020; ||Y||_2^2 is the Gram's
+    largest eigenvalue). That bound assumes every product of the length-N
+    inner products is summed in FP32; at N = 1e7 it exceeds ||Y||_2^2 itself,
+    so the factor stops preconditioning and the FP64 pass that follows cannot
+    reach FP64 orthogonality. shift="adaptive" (the default) tries the
+    unshifted FP32 factorization first and falls back to the Fukaya shift
+    only when it fails, as the reference does for its FP64 rule
+    (src/panelblas.jl:148-155). This pass has no counterpart in the reference."""
     G = np.asarray(G)
+    narrow = np.complex64 if np.iscomplexobj(G) else np.float32
+    if shift == "adaptive":
+        R = _potrf(G.astype(narrow))
+        if R is not None:
+            return R, 0.0
+    elif shift != "fukaya":
+        raise ArgumentError(f"shift must be 'adaptive' or 'fukaya', got {shift!r}")
     lam = scipy.linalg.eigh(G, eigvals_only=True, subset_by_index=[k - 1, k - 1])[0]
-    shift = 11.0 * (N * k + k * (k + 1)) * UNIT_ROUNDOFF_FP32 * lam
-    G32 = (G + shift * np.eye(k)).astype(np.complex64 if np.iscomplexobj(G) else np.float32)
-    R = _potrf(G32)
+    s = 11.0 * (N * k + k * (k + 1)) * UNIT_ROUNDOFF_FP32 * lam
+    R = _potrf((G + s * np.eye(k)).astype(narrow))
     if R is None:
         raise ArgumentError(
-            f"the {k}×{k} FP32 Gram matrix is not positive definite after the shift {shift}; the columns are "
+            f"the {k}×{k} FP32 Gram matrix is not positive definite after the shift {s}; the columns are "
             "numerically dependent")
-    return R, shift
+    return R, s
 
 
-def cholqr_pass_mixed(Y, nbuffers=None):
+def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive"):
     """The mixed first pass: FP32 matrix-core Gram with FP64 sums across row
-    chunks, shifted FP32 Cholesky, FP32 triangular solve on the FP64 data."""
-    narrow = np.complex64 if np.issubdtype(Y.dtype, np.complexfloating) else np.float32
-    if Y.dtype not in (np.dtype(np.complex128),):
+    chunks, FP32 Cholesky (shifted when needed), FP32 triangular solve on the
+    FP64 data. Returns (R, shift)."""
+    if Y.dtype != np.dtype(np.complex128):
         raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
-    G = gram(Y, nbuffers=nbuffers, compute=narrow)
-    R32, _ = cholesky_upper_shifted_fp32(G, Y.N, Y.k)
+    G = gram(Y, nbuffers=nbuffers, compute=np.complex64)
+    R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
     R = R32.astype(Y.dtype)
-    rdiv_rows(Y, R, nbuffers=nbuffers, compute=narrow)
-    return R
+    rdiv_rows(Y, R, nbuffers=nbuffers, compute=np.complex64)
+    return R, s
 
 
-def cholqr2_mixed(Y, *, nbuffers=None):
-    """cholqr2 with the first pass in mixed precision (FP32 Gram and solve,
-    shifted) as a preconditioner and the second pass in FP64. A third FP64
-    pass runs when the second one's Cholesky needed the reference's shift."""
-    R1 = cholqr_pass_mixed(Y, nbuffers)
+def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive"):
+    """cholqr2 with the first pass in mixed precision (FP32 Gram and solve) as
+    a preconditioner and the second pass in FP64. A third FP64 pass runs when
+    the second one's Cholesky needed the reference's shift."""
+    R1, _ = cholqr_pass_mixed(Y, nbuffers, shift)
     R2, shifted = cholqr_pass(Y, nbuffers)
     R = R2 @ R1
     if shifted:

@@ -191,6 +191,12 @@ int fhip_gemm_nn(void *D, int64_t ldd, const void *A, int64_t lda, const void *C
 int fhip_convert_copy(void *dst, int dst_type, int64_t ldd, const void *src, int src_type,
                       int64_t lds, int64_t rows, int64_t cols, void *stream);
 
+/* Y <- a X + b Y for rows x cols matrices of one eltype (b = 0 still reads Y).
+ * The elementwise step of the reference's axpy!/scale! (src/panelblas.jl:
+ * 311-331) and of the residual check Y - Q R. */
+int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t rows, int64_t cols,
+               double a_re, double a_im, double b_re, double b_im, int type, void *stream);
+
 /* dst <- value for every entry of a rows x cols matrix (value_im ignored for
  * real types). A kernel rather than a memset, so it is safe under capture. */
 int fhip_fill(void *dst, int type, int64_t ld, int64_t rows, int64_t cols, double value_re,
