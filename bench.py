@@ -96,6 +96,22 @@ class KernelTimer:
         return out
 
 
+def measured_traffic(kind):
+    """HBM bytes per launch of the configuration-2 row-block kernels, as the
+    FETCH_SIZE / WRITE_SIZE counter runs measured them (scaled by a plain copy
+    kernel); the summary, with its commands, is in
+    profiles/hbm_traffic_rowblock_629856x512.md."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_rowblock_629856x512.md")
+    try:
+        for line in open(path):
+            if line.startswith("machine-readable:"):
+                fields = dict(tok.split("=") for tok in line.split(":", 1)[1].split())
+                return int(fields[f"{kind}_traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def gram_flops(rows, k):
     """LAPACK Working Note 41 count for a Hermitian k x k Gram over `rows` rows."""
     return 4.0 * rows * k * (k + 1)
@@ -167,6 +183,31 @@ def residual(backend, Y, R, G, omega):
     return float(np.sqrt(num / den))
 
 
+def kernels_only(rows, k, reps):
+    """The per-launch shapes of the benchmark, without the pipeline around
+    them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
+    for the HBM counters), then `reps` FP64 Grams, FP32 (mixed) Grams, FP64 and
+    FP32 triangular solves on it. For profiling runs."""
+    import funicular_hip as F
+    backend = F.hip_backend()
+    A = backend.alloc_device(np.complex128, (rows, k))
+    B = backend.alloc_device(np.complex128, (rows, k))
+    C = backend.alloc_device(np.complex128, (k, k))
+    backend.ghost_fill(A, 0, 1, 7)
+    backend.convert_copy(B, A)
+    backend.sync_all()
+    for compute in (None, np.complex64):
+        for _ in range(reps):
+            backend.gram_accumulate(C, A, A, 1.0, 0.0, compute=compute)
+    G = backend.to_host(C)
+    R = backend.todevice(np.asfortranarray(np.linalg.cholesky(G).conj().T))
+    for compute in (None, np.complex64):
+        for _ in range(reps):
+            backend.rdiv_upper(B, R, compute=compute)
+    backend.sync_all()
+    log(f"kernels-only: {rows}x{k}, {reps} repetitions of each kernel")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,7 +219,12 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profile mode: only the per-launch kernels at the row-block shape")
     args = ap.parse_args()
+    if args.kernels_only:
+        kernels_only(args.grid ** 3 * args.w // args.k, args.k, max(args.steps, 1))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -246,7 +292,8 @@ def main():
                    "N": N, "k": k, "panel_width": w, "row_block": F.row_block_height(N, k, w)},
         "roofline": {"bound": "mfma", "kernel": "gram_partial_kernel + gram_reduce_kernel (FP64)",
                      "achieved": gram64["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": gram64["tflops"] / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": gram64["tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic("gram_fp64") if (N, k, w) == (10077696, 512, 32) else None,
                      "avg_launch_ms": gram64["avg_ms"], "flops_per_launch": gram64["flops_per_launch"]},
         "trsm": {"kernel": "trsm_diag_inv_kernel + trsm_runn_kernel (FP64)", "achieved": trsm64["tflops"],
                  "peak": FP64_PEAK_TFLOPS, "frac": trsm64["tflops"] / FP64_PEAK_TFLOPS,

@@ -36,9 +36,9 @@ def sources():
     )
 
 
-def compile_one(src):
-    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+def compile_one(src, extra=(), obj_dir=None):
+    obj = os.path.join(obj_dir or OBJ_DIR, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-c", src, "-o", obj]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     # hipcc's host pass does not know the packed-fp32 feature; that warning is harmless.
     noise = "is not a recognized feature for this target"
@@ -48,23 +48,28 @@ def compile_one(src):
     return obj, err
 
 
-def build(verbose=False):
-    os.makedirs(OBJ_DIR, exist_ok=True)
+def build(verbose=False, defines=(), out=None):
+    """Compiles every source (with optional -D tuning defines) and links the
+    library, by default in place of funicular_hip/libfunicular_hip.so."""
+    obj_dir = OBJ_DIR if out is None else out + ".objs"
+    os.makedirs(obj_dir, exist_ok=True)
+    extra = [f"-D{d}" for d in defines]
     srcs = sources()
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        results = list(ex.map(compile_one, srcs))
+        results = list(ex.map(lambda s: compile_one(s, extra, obj_dir), srcs))
     objs = [o for o, _ in results]
     if verbose:
         for (o, err) in results:
             if err.strip():
                 print(err, file=sys.stderr)
-    tmp = LIB + ".tmp"
+    lib = LIB if out is None else out
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

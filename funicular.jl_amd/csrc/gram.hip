@@ -33,12 +33,22 @@ struct GramShape<128> {
   static constexpr int MB = 4, NB = 4;
   static constexpr int RESIDENT = 512;
 };
+#ifndef FHIP_GRAM64_KC
+#define FHIP_GRAM64_KC 16
+#endif
+#ifndef FHIP_GRAM64_RESIDENT
+#define FHIP_GRAM64_RESIDENT 1024
+#endif
+#ifndef FHIP_GRAM_FP32_TILE
+#define FHIP_GRAM_FP32_TILE 64
+#endif
+
 template <>
 struct GramShape<64> {
-  static constexpr int KC = 16;
+  static constexpr int KC = FHIP_GRAM64_KC;
   static constexpr int KSPLIT = 1;
   static constexpr int MB = 2, NB = 2;
-  static constexpr int RESIDENT = 1024;
+  static constexpr int RESIDENT = FHIP_GRAM64_RESIDENT;
 };
 template <>
 struct GramShape<32> {
@@ -78,7 +88,7 @@ static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm, bool fp3
   GramPlan p;
   // FP32 products run at twice the FP64 rate, so the single-precision Gram
   // takes 128-wide tiles to halve the operand traffic per flop.
-  p.tile = (ka > 64 && kb > 64 && fp32) ? 128 : (ka > 32 || kb > 32) ? 64 : 32;
+  p.tile = (ka > 64 && kb > 64 && fp32 && FHIP_GRAM_FP32_TILE == 128) ? 128 : (ka > 32 || kb > 32) ? 64 : 32;
   int kc, resident;
   if (p.tile == 128) {
     kc = GramShape<128>::KC; p.ksplit = GramShape<128>::KSPLIT; resident = GramShape<128>::RESIDENT;

@@ -122,8 +122,11 @@ __global__ __launch_bounds__(NN_CB) void trsm_diag_inv_kernel(const TS *__restri
 // the 16x16 matrix-core tile. The update streams TR_KC columns of the solved
 // rows and of R at a time, the next slice's loads in flight during the
 // current slice's products.
+#ifndef FHIP_TRSM_KC
+#define FHIP_TRSM_KC 16
+#endif
 constexpr int TR_RW = 128;
-constexpr int TR_KC = 16;
+constexpr int TR_KC = FHIP_TRSM_KC;
 constexpr int TR_LDA = TR_RW + 4;
 constexpr int TR_LDB = NN_CB + 4;
 constexpr int TR_XPT = TR_KC * TR_RW / NN_THREADS;  // X elements per thread per slice (8)
