@@ -157,9 +157,57 @@ __global__ __launch_bounds__(256) void axpby_kernel(T *__restrict__ Y, int64_t l
   }
 }
 
+// out[i] (+)= Re sum_j conj(A[i, j]) B[i, j]: one thread per row, columns
+// in order, so the sum is deterministic.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void rowdot_kernel(double *__restrict__ out, const T *__restrict__ A,
+                                                     int64_t lda, const T *__restrict__ B, int64_t ldb,
+                                                     int64_t rows, int64_t cols, int accumulate) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  double s = accumulate ? out[i] : 0.0;
+  for (int64_t j = 0; j < cols; ++j) {
+    T ar, ai, br, bi;
+    load_elem<T, T, C>(A, i + j * lda, true, ar, ai);
+    load_elem<T, T, C>(B, i + j * ldb, true, br, bi);
+    s += (double)ar * (double)br + (double)ai * (double)bi;
+  }
+  out[i] = s;
+}
+
 }  // namespace fhip
 
 using namespace fhip;
+
+extern "C" int fhip_rowdot(double *out, const void *A, int64_t lda, const void *B, int64_t ldb, int64_t rows,
+                           int64_t cols, int type, int accumulate, void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_rowdot: unknown eltype");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_rowdot: negative dimension");
+  if (rows == 0) return FHIP_OK;
+  if (!out || (cols > 0 && (!A || !B))) return fail(FHIP_E_ARGUMENT, "fhip_rowdot: null pointer");
+  if (lda < rows || ldb < rows) return fail(FHIP_E_ARGUMENT, "fhip_rowdot: leading dimension smaller than the rows");
+  const unsigned g = (unsigned)cdiv64(rows, 256);
+  hipStream_t s = (hipStream_t)stream;
+  switch (type) {
+    case FHIP_F32:
+      hipLaunchKernelGGL((rowdot_kernel<float, false>), dim3(g), dim3(256), 0, s, out, (const float *)A, lda,
+                         (const float *)B, ldb, rows, cols, accumulate);
+      break;
+    case FHIP_F64:
+      hipLaunchKernelGGL((rowdot_kernel<double, false>), dim3(g), dim3(256), 0, s, out, (const double *)A, lda,
+                         (const double *)B, ldb, rows, cols, accumulate);
+      break;
+    case FHIP_C32:
+      hipLaunchKernelGGL((rowdot_kernel<float, true>), dim3(g), dim3(256), 0, s, out, (const float *)A, lda,
+                         (const float *)B, ldb, rows, cols, accumulate);
+      break;
+    case FHIP_C64:
+      hipLaunchKernelGGL((rowdot_kernel<double, true>), dim3(g), dim3(256), 0, s, out, (const double *)A, lda,
+                         (const double *)B, ldb, rows, cols, accumulate);
+      break;
+  }
+  return check_launch("rowdot_kernel");
+}
 
 extern "C" int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t rows, int64_t cols,
                           double a_re, double a_im, double b_re, double b_im, int type, void *stream) {

@@ -132,10 +132,12 @@ constexpr int TR_LDB = NN_CB + 4;
 constexpr int TR_XPT = TR_KC * TR_RW / NN_THREADS;  // X elements per thread per slice (8)
 constexpr int TR_RPT = TR_KC * NN_CB / NN_THREADS;  // R elements per thread per slice (2)
 
-template <typename TS, typename T, bool CPLX>
-__global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void trsm_runn_kernel(TS *__restrict__ Y, int64_t ldy, int64_t m,
-                                                                int k, const TS *__restrict__ R, int64_t ldr,
-                                                                const T *__restrict__ Dinv) {
+// CT = false solves X R = Y (left to right over column blocks); CT = true
+// solves X R^H = Y, whose factor R^H is lower triangular, right to left.
+template <typename TS, typename T, bool CPLX, bool CT>
+__global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void trsm_runn_kernel(
+    TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const TS *__restrict__ R, int64_t ldr,
+    const T *__restrict__ Dinv) {
   constexpr int NP = CPLX ? 2 : 1;
   __shared__ T As[NP][TR_KC * TR_LDA];
   __shared__ T Bs[NP][TR_KC * TR_LDB];
@@ -159,14 +161,21 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
       const int r = idx % TR_RW;
       const int kk = idx / TR_RW;
       const int64_t row = row0 + r;
-      load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, row < m, xr[e], xi[e]);
+      load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, row < m && kc + kk < k, xr[e], xi[e]);
     }
 #pragma unroll
     for (int e = 0; e < TR_RPT; ++e) {
       const int idx = t + NN_THREADS * e;
       const int kk = idx % TR_KC;
       const int c = idx / TR_KC;
-      load_elem<TS, T, CPLX>(R, (int64_t)(kc + kk) + (int64_t)(c0 + c) * ldr, c0 + c < k, rr[e], ri[e]);
+      if (CT) {
+        // Element (kc + kk, c0 + c) of R^H is conj(R[c0 + c, kc + kk]).
+        load_elem<TS, T, CPLX>(R, (int64_t)(c0 + c) + (int64_t)(kc + kk) * ldr, kc + kk < k && c0 + c < k,
+                               rr[e], ri[e]);
+        ri[e] = -ri[e];
+      } else {
+        load_elem<TS, T, CPLX>(R, (int64_t)(kc + kk) + (int64_t)(c0 + c) * ldr, c0 + c < k, rr[e], ri[e]);
+      }
     }
   };
   auto stash = [&]() {
@@ -188,7 +197,8 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
     }
   };
 
-  for (int J = 0; J < nbk; ++J) {
+  for (int step = 0; step < nbk; ++step) {
+    const int J = CT ? nbk - 1 - step : step;
     const int c0 = J * NN_CB;
     acc_t acc_re[2][2], acc_im[2][2];
 #pragma unroll
@@ -204,13 +214,16 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
           acc_re[mb][nb][r] = re;
           acc_im[mb][nb][r] = im;
         }
-    // Trailing update from the blocks already solved.
-    if (c0 > 0) fetch(0, c0);
-    for (int kc = 0; kc < c0; kc += TR_KC) {
+    // Update from the blocks already solved: those left of J for R, right of
+    // J for R^H.
+    const int kbeg = CT ? c0 + NN_CB : 0;
+    const int kend = CT ? k : c0;
+    if (kbeg < kend) fetch(kbeg, c0);
+    for (int kc = kbeg; kc < kend; kc += TR_KC) {
       __syncthreads();
       stash();
       __syncthreads();
-      if (kc + TR_KC < c0) fetch(kc + TR_KC, c0);
+      if (kc + TR_KC < kend) fetch(kc + TR_KC, c0);
       mma_steps<T, CPLX, 2, 2, PROD_SUB>(As[0], As[NP - 1], TR_LDA, mi0, Bs[0], Bs[NP - 1], TR_LDB, 0, 0,
                                          TR_KC / 4, 1, acc_re, acc_im);
     }
@@ -241,7 +254,13 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
         const int kk = idx % TR_KC;
         const int c = idx / TR_KC;
         T re, im;
-        load_elem<T, T, CPLX>(Dj, (int64_t)(16 * h + kk) + (int64_t)c * NN_CB, true, re, im);
+        if (CT) {
+          // inv(R_JJ^H) = inv(R_JJ)^H.
+          load_elem<T, T, CPLX>(Dj, (int64_t)c + (int64_t)(16 * h + kk) * NN_CB, true, re, im);
+          im = -im;
+        } else {
+          load_elem<T, T, CPLX>(Dj, (int64_t)(16 * h + kk) + (int64_t)c * NN_CB, true, re, im);
+        }
         Bs[0][kk * TR_LDB + c] = re;
         if (CPLX) Bs[NP - 1][kk * TR_LDB + c] = im;
       }
@@ -318,7 +337,7 @@ __global__ __launch_bounds__(NN_THREADS) void gemm_nn_kernel(T *__restrict__ D, 
     }
 }
 
-template <typename TS, typename T, bool CPLX>
+template <typename TS, typename T, bool CPLX, bool CT = false>
 static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_t ldr, T *Dinv,
                        hipStream_t stream) {
   const int nbk = (k + NN_CB - 1) / NN_CB;
@@ -326,7 +345,7 @@ static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_
   int st = check_launch("trsm_diag_inv_kernel");
   if (st || m == 0) return st;
   const int64_t grid = cdiv64(m, TR_RW);
-  hipLaunchKernelGGL((trsm_runn_kernel<TS, T, CPLX>), dim3((unsigned)grid), dim3(NN_THREADS), 0, stream, Y, ldy,
+  hipLaunchKernelGGL((trsm_runn_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(NN_THREADS), 0, stream, Y, ldy,
                      m, k, R, ldr, (const T *)Dinv);
   return check_launch("trsm_runn_kernel");
 }
@@ -387,6 +406,35 @@ extern "C" int fhip_trsm_runn(void *Y, int64_t ldy, int64_t m, int64_t k, const 
       return launch_trsm<float, float, false>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)work, s);
   }
   return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runn: unknown type %d", type);
+}
+
+extern "C" int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr, int type,
+                              void *work, size_t work_bytes, void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runc: unknown type %d", type);
+  if (m < 0 || k < 0) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: negative dimension");
+  if (k == 0) return FHIP_OK;
+  if (k > (1 << 20)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: too many columns");
+  if (!Y || !R) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: null matrix pointer");
+  if (ldr < k || (m > 0 && ldy < m))
+    return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: leading dimension smaller than the matrix");
+  size_t need = 0;
+  fhip_trsm_workspace_bytes(&need, k, type);
+  if (!work || work_bytes < need)
+    return fail(FHIP_E_WORKSPACE, "fhip_trsm_runc: workspace of %zu bytes is smaller than the %zu needed",
+                work_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  const int ik = (int)k;
+  switch (type) {
+    case FHIP_C64:
+      return launch_trsm<double, double, true, true>((double *)Y, ldy, m, ik, (const double *)R, ldr, (double *)work, s);
+    case FHIP_C32:
+      return launch_trsm<float, float, true, true>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)work, s);
+    case FHIP_F64:
+      return launch_trsm<double, double, false, true>((double *)Y, ldy, m, ik, (const double *)R, ldr, (double *)work, s);
+    case FHIP_F32:
+      return launch_trsm<float, float, false, true>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)work, s);
+  }
+  return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runc: unknown type %d", type);
 }
 
 extern "C" int fhip_gemm_nn(void *D, int64_t ldd, const void *A, int64_t lda, const void *C, int64_t ldc,

@@ -13,6 +13,7 @@ _lib.load()
 
 from .device_api import (ArgumentError, DeviceArray, DimensionMismatch, HIPBackend, hip_backend,  # noqa: E402
                          human_readable_bytes, slab_matrix)
+from .adjoint import rrule_cholqr2, rrule_project  # noqa: E402
 from .executor import (PanelSteps, RowSteps, foreachpanel, readpanels, rowtraversal, sweep,  # noqa: E402
                        updatepanels, writepanels)
 from .ghosts import GhostPanels  # noqa: E402
@@ -28,4 +29,5 @@ from .plan import (ResidencyPlan, choose_panel_width, device_bytes_allocated, re
 __all__ = [
     "GhostPanels", "PanelMatrix", "ResidencyPlan", "check_operator", "cholqr2", "copycols", "foreachpanel",
     "gram", "npanels", "panelmul", "panelrange", "panelwidth", "project", "rightmul", "hip_backend",
+    "rrule_cholqr2", "rrule_project",
 ]

@@ -81,9 +81,11 @@ _SIGNATURES = {
     "fhip_gram": [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _d, _d, _d, _d, _i, _i, _vp, _sz, _vp],
     "fhip_trsm_workspace_bytes": [ctypes.POINTER(_sz), _i64, _i],
     "fhip_trsm_runn": [_vp, _i64, _i64, _i64, _vp, _i64, _i, _i, _vp, _sz, _vp],
+    "fhip_trsm_runc": [_vp, _i64, _i64, _i64, _vp, _i64, _i, _vp, _sz, _vp],
     "fhip_gemm_nn": [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i, _vp],
     "fhip_convert_copy": [_vp, _i, _i64, _vp, _i, _i64, _i64, _i64, _vp],
     "fhip_fill": [_vp, _i, _i64, _i64, _i64, _d, _d, _vp],
+    "fhip_rowdot": [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i, _i, _vp],
     "fhip_axpby": [_vp, _i64, _vp, _i64, _i64, _i64, _d, _d, _d, _d, _i, _vp],
     "fhip_ghost_randn": [_vp, _i, _i64, _i64, _i64, _i64, _i64, _u64, _vp],
     "fhip_ghost_tables_checksum": [ctypes.POINTER(_u64)],

@@ -513,6 +513,18 @@ class HIPBackend(DeviceBackend):
              work_ptr, work_bytes, current_stream())
         return Y
 
+    def rdiv_upper_adjoint(self, Y, R, work=None):
+        """Y <- Y R^-H for upper triangular R (the solve of the QR adjoint)."""
+        if not (R.rows == R.cols == Y.cols):
+            raise DimensionMismatch(f"a {Y.shape} block cannot be divided by a {R.shape} triangular factor")
+        if Y.size == 0:
+            return Y
+        need = self.trsm_workspace_bytes(R.rows, Y.dtype)
+        work_ptr, work_bytes = _workspace(self, work, need)
+        call("fhip_trsm_runc", Y.ptr, Y.ld, Y.rows, Y.cols, R.ptr, R.ld, eltype_code(Y.dtype), work_ptr,
+             work_bytes, current_stream())
+        return Y
+
     def convert_copy(self, dst, src, stream=None):
         """dst <- convert(eltype(dst), src) (src/kernels.jl:14-20)."""
         if tuple(dst.shape) != tuple(src.shape):
@@ -532,6 +544,14 @@ class HIPBackend(DeviceBackend):
         call("fhip_axpby", Y.ptr, Y.ld, X.ptr, X.ld, Y.rows, Y.cols, a.real, a.imag, b.real, b.imag,
              eltype_code(Y.dtype), current_stream())
         return Y
+
+    def rowdot(self, out, A, B, accumulate=False):
+        """out[i] (+)= Re sum_j conj(A[i, j]) B[i, j]; out is a float64 device vector."""
+        if tuple(A.shape) != tuple(B.shape) or A.dtype != B.dtype or out.rows < A.rows:
+            raise DimensionMismatch(f"rowdot of {A.shape} and {B.shape} into {out.shape}")
+        call("fhip_rowdot", out.ptr, A.ptr, A.ld, B.ptr, B.ld, A.rows, A.cols, eltype_code(A.dtype),
+             1 if accumulate else 0, current_stream())
+        return out
 
     def fill(self, dst, value):
         v = complex(value)

@@ -181,6 +181,14 @@ int fhip_trsm_runn(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R,
                    int64_t ldr, int type, int compute, void *work, size_t work_bytes,
                    void *stream);
 
+/* Y <- Y * R^-H in place (right side, R upper triangular k x k, conjugate
+ * transpose): the solve of the QR adjoint, dY = (dQ + Q N) R^-H. Same
+ * blocking as fhip_trsm_runn, walking the column blocks right to left. No
+ * counterpart in the reference (its cholqr2! has no adjoint); the workspace
+ * is fhip_trsm_workspace_bytes(k, type). */
+int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr,
+                   int type, void *work, size_t work_bytes, void *stream);
+
 /* D <- A * C, A m x s (lda), C s x r (ldc), D m x r (ldd). D must not alias A.
  * Replaces rightmul_gemm! (src/kernels.jl:73; semantics src/kernels.jl:57-64). */
 int fhip_gemm_nn(void *D, int64_t ldd, const void *A, int64_t lda, const void *C,
@@ -196,6 +204,13 @@ int fhip_convert_copy(void *dst, int dst_type, int64_t ldd, const void *src, int
  * 311-331) and of the residual check Y - Q R. */
 int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t rows, int64_t cols,
                double a_re, double a_im, double b_re, double b_im, int type, void *stream);
+
+/* out[i] (+)= Re sum_j conj(A[i, j]) B[i, j] for i < rows (out is FP64,
+ * overwritten unless accumulate = 1): the per-row reduction behind the
+ * sensitivity of S = Q^H G Q to the operator's diagonal, dL/dd_i =
+ * Re sum_a conj(Q_ia) (Q dS^H)_ia. No counterpart in the reference. */
+int fhip_rowdot(double *out, const void *A, int64_t lda, const void *B, int64_t ldb, int64_t rows,
+                int64_t cols, int type, int accumulate, void *stream);
 
 /* dst <- value for every entry of a rows x cols matrix (value_im ignored for
  * real types). A kernel rather than a memset, so it is safe under capture. */
