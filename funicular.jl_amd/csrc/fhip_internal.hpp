@@ -121,6 +121,70 @@ __device__ inline void mma_steps(const T *__restrict__ As_re, const T *__restric
   }
 }
 
+// The same K steps for complex operands with three real products per complex
+// product instead of four (the 3M method; Higham, "Stability of a method for
+// multiplying complex matrices with three real matrix multiplications", SIMAX
+// 1992). Per element pair the products are
+//     p1 += ar br,   p2 += ai bi,   p3 += (ar + ai) (br -/+ bi),
+// the sign being - for conj(a) b and + for a b; combine3m() turns the three
+// sums into the real and imaginary parts. PROD_SUB negates a, so the sums
+// count down from whatever the accumulators started at. The error bound is
+// normwise, |error| <= c u |a| |b| per product, the same order as the
+// four-product form's for these column-scaled sums.
+template <typename T, int MB, int NB, int MODE>
+__device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__restrict__ As_im, int lda_s,
+                                    int mi0, const T *__restrict__ Bs_re, const T *__restrict__ Bs_im,
+                                    int ldb_s, int nj0, int s_begin, int s_end, int s_stride,
+                                    typename Mfma<T>::acc (&p1)[MB][NB], typename Mfma<T>::acc (&p2)[MB][NB],
+                                    typename Mfma<T>::acc (&p3)[MB][NB]) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15;
+  const int lk = lane >> 4;
+  for (int s = s_begin; s < s_end; s += s_stride) {
+    const int kr = 4 * s + lk;
+    T ar[MB], ai[MB], as[MB], br[NB], bi[NB], bt[NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) {
+      ar[mb] = As_re[kr * lda_s + mi0 + 16 * mb + li];
+      ai[mb] = As_im[kr * lda_s + mi0 + 16 * mb + li];
+      as[mb] = ar[mb] + ai[mb];
+      if (MODE == PROD_SUB) {
+        ar[mb] = -ar[mb];
+        ai[mb] = -ai[mb];
+        as[mb] = -as[mb];
+      }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      br[nb] = Bs_re[kr * ldb_s + nj0 + 16 * nb + li];
+      bi[nb] = Bs_im[kr * ldb_s + nj0 + 16 * nb + li];
+      bt[nb] = (MODE == PROD_CONJ_A) ? br[nb] - bi[nb] : br[nb] + bi[nb];
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        p1[mb][nb] = Mfma<T>::mma(ar[mb], br[nb], p1[mb][nb]);
+        p2[mb][nb] = Mfma<T>::mma(ai[mb], bi[nb], p2[mb][nb]);
+        p3[mb][nb] = Mfma<T>::mma(as[mb], bt[nb], p3[mb][nb]);
+      }
+  }
+}
+
+// Real and imaginary parts from the three sums of mma_steps_3m:
+//   conj(a) b:  re = p1 + p2,  im = p1 - p2 - p3;
+//   a b, -a b:  re = p1 - p2,  im = p3 - p1 - p2.
+template <int MODE, typename V>
+__device__ inline void combine3m(const V &p1, const V &p2, const V &p3, V &re, V &im) {
+  if (MODE == PROD_CONJ_A) {
+    re = p1 + p2;
+    im = (p1 - p2) - p3;
+  } else {
+    re = p1 - p2;
+    im = p3 - (p1 + p2);
+  }
+}
+
 template <typename T>
 struct Pair;
 template <>

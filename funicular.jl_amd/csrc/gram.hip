@@ -7,7 +7,8 @@
 //    A and B are the same matrix) and the m rows into chunks. One workgroup
 //    owns one (tile, chunk) pair and sums its rows with v_mfma_f64_16x16x4
 //    (or the f32 form for the mixed pass) on the real and imaginary planes,
-//    staging KC-row slices of both operands through LDS.
+//    staging KC-row slices of both operands through LDS. A complex product
+//    takes three real matrix-core products (3M), not four.
 //  * Workgroup ids are dealt round-robin to the 8 XCDs, so the id is split as
 //    (xcd, local) and every tile of one chunk runs on the same XCD: the rows
 //    of a chunk come from HBM once and the other tiles hit that XCD's L2.
@@ -168,13 +169,15 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
   const int s_stride = KSPLIT;
 
   typedef typename Mfma<TC>::acc acc_t;
-  acc_t acc_re[MB][NB], acc_im[MB][NB];
+  // Complex tiles keep the three 3M sums (p1, p2, p3); real tiles use p1 only.
+  acc_t p1[MB][NB], p2[MB][NB], p3[MB][NB];
 #pragma unroll
   for (int a = 0; a < MB; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      acc_re[a][b] = Mfma<TC>::zero();
-      acc_im[a][b] = Mfma<TC>::zero();
+      p1[a][b] = Mfma<TC>::zero();
+      p2[a][b] = Mfma<TC>::zero();
+      p3[a][b] = Mfma<TC>::zero();
     }
 
   // Register staging of the next slice while the current one is multiplied.
@@ -213,8 +216,12 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
     if (rb + KC < r1) fetch(rb + KC);
     const TC *bre = same ? As[0] : Bs[0];
     const TC *bim = same ? As[NP - 1] : Bs[NP - 1];
-    mma_steps<TC, CPLX, MB, NB, PROD_CONJ_A>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
-                                           KC / 4, s_stride, acc_re, acc_im);
+    if constexpr (CPLX)
+      mma_steps_3m<TC, MB, NB, PROD_CONJ_A>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
+                                            KC / 4, s_stride, p1, p2, p3);
+    else
+      mma_steps<TC, false, MB, NB, PROD_CONJ_A>(As[0], As[0], LDS_W, mi0, bre, bre, LDS_W, nj0, s_begin,
+                                                KC / 4, s_stride, p1, p2);
   }
 
   // Partial slot of this workgroup (and wave, when waves split K).
@@ -223,15 +230,18 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
+    for (int nb = 0; nb < NB; ++nb) {
+      acc_t re = p1[mb][nb], im = p2[mb][nb];
+      if constexpr (CPLX) combine3m<PROD_CONJ_A>(p1[mb][nb], p2[mb][nb], p3[mb][nb], re, im);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int il = mi0 + 16 * mb + Mfma<TC>::row(lk, r);
         const int jl = nj0 + 16 * nb + li;
         const int64_t off = ((int64_t)jl * TILE + il) * NP;
-        out[off] = acc_re[mb][nb][r];
-        if (CPLX) out[off + 1] = acc_im[mb][nb][r];
+        out[off] = re[r];
+        if (CPLX) out[off + 1] = im[r];
       }
+    }
 }
 
 // Adds the partial slots in slot order and applies alpha and beta.

@@ -200,7 +200,10 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
   for (int step = 0; step < nbk; ++step) {
     const int J = CT ? nbk - 1 - step : step;
     const int c0 = J * NN_CB;
-    acc_t acc_re[2][2], acc_im[2][2];
+    // The block's right-hand side seeds the update sums. Complex blocks keep
+    // the three 3M sums: p1 = Y_re, p2 = 0, p3 = Y_re + Y_im, counted down by
+    // the products, so that combine3m yields Y - sum X_I R_IJ directly.
+    acc_t p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -211,8 +214,9 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
           const int col = c0 + 16 * nb + li;
           T re, im;
           load_elem<TS, T, CPLX>(Y, row + (int64_t)col * ldy, row < m && col < k, re, im);
-          acc_re[mb][nb][r] = re;
-          acc_im[mb][nb][r] = im;
+          p1[mb][nb][r] = re;
+          p2[mb][nb][r] = CPLX ? T(0) : im;
+          p3[mb][nb][r] = re + im;
         }
     // Update from the blocks already solved: those left of J for R, right of
     // J for R^H.
@@ -224,10 +228,25 @@ __global__ __launch_bounds__(NN_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
       stash();
       __syncthreads();
       if (kc + TR_KC < kend) fetch(kc + TR_KC, c0);
-      mma_steps<T, CPLX, 2, 2, PROD_SUB>(As[0], As[NP - 1], TR_LDA, mi0, Bs[0], Bs[NP - 1], TR_LDB, 0, 0,
-                                         TR_KC / 4, 1, acc_re, acc_im);
+      if constexpr (CPLX)
+        mma_steps_3m<T, 2, 2, PROD_SUB>(As[0], As[NP - 1], TR_LDA, mi0, Bs[0], Bs[NP - 1], TR_LDB, 0, 0,
+                                        TR_KC / 4, 1, p1, p2, p3);
+      else
+        mma_steps<T, false, 2, 2, PROD_SUB>(As[0], As[0], TR_LDA, mi0, Bs[0], Bs[0], TR_LDB, 0, 0, TR_KC / 4,
+                                            1, p1, p2);
     }
-    // Multiply by the inverse of the diagonal block, one 16-column half at a time.
+    acc_t acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        acc_re[mb][nb] = p1[mb][nb];
+        acc_im[mb][nb] = p2[mb][nb];
+        if constexpr (CPLX) combine3m<PROD_ADD>(p1[mb][nb], p2[mb][nb], p3[mb][nb], acc_re[mb][nb], acc_im[mb][nb]);
+      }
+    // Multiply by the inverse of the diagonal block, one 16-column half at a
+    // time. This product is an eighth of the work at k = 512; it keeps the
+    // four-product form, which needs two accumulator sets instead of three.
     acc_t x_re[2][2], x_im[2][2];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
@@ -313,28 +332,36 @@ __global__ __launch_bounds__(NN_THREADS) void gemm_nn_kernel(T *__restrict__ D, 
   const int lk = lane >> 4;
   const int mi0 = 16 * w;
 
-  acc_t acc_re[1][2], acc_im[1][2];
+  acc_t p1[1][2], p2[1][2], p3[1][2];
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
-    acc_re[0][nb] = Mfma<T>::zero();
-    acc_im[0][nb] = Mfma<T>::zero();
+    p1[0][nb] = Mfma<T>::zero();
+    p2[0][nb] = Mfma<T>::zero();
+    p3[0][nb] = Mfma<T>::zero();
   }
   for (int kc = 0; kc < s; kc += NN_KC) {
     __syncthreads();
     stage_rows<T, T, CPLX>(As[0], As[NP - 1], A, lda, row0, m, kc, s);
     stage_cols<T, T, CPLX>(Bs[0], Bs[NP - 1], C, ldc, kc, s, c0, r);
     __syncthreads();
-    mma_steps<T, CPLX, 1, 2, PROD_ADD>(As[0], As[NP - 1], NN_LDA, mi0, Bs[0], Bs[NP - 1], NN_LDB, 0, 0,
-                                       NN_KC / 4, 1, acc_re, acc_im);
+    if constexpr (CPLX)
+      mma_steps_3m<T, 1, 2, PROD_ADD>(As[0], As[NP - 1], NN_LDA, mi0, Bs[0], Bs[NP - 1], NN_LDB, 0, 0, NN_KC / 4,
+                                      1, p1, p2, p3);
+    else
+      mma_steps<T, false, 1, 2, PROD_ADD>(As[0], As[0], NN_LDA, mi0, Bs[0], Bs[0], NN_LDB, 0, 0, NN_KC / 4, 1,
+                                          p1, p2);
   }
 #pragma unroll
-  for (int nb = 0; nb < 2; ++nb)
+  for (int nb = 0; nb < 2; ++nb) {
+    acc_t re = p1[0][nb], im = p2[0][nb];
+    if constexpr (CPLX) combine3m<PROD_ADD>(p1[0][nb], p2[0][nb], p3[0][nb], re, im);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t row = row0 + mi0 + Mfma<T>::row(lk, q);
       const int col = c0 + 16 * nb + li;
-      if (row < m && col < r) store_elem<T, CPLX>(D, row + (int64_t)col * ldd, acc_re[0][nb][q], acc_im[0][nb][q]);
+      if (row < m && col < r) store_elem<T, CPLX>(D, row + (int64_t)col * ldd, re[q], im[q]);
     }
+  }
 }
 
 template <typename TS, typename T, bool CPLX, bool CT = false>
