@@ -183,6 +183,94 @@ def residual(backend, Y, R, G, omega):
     return float(np.sqrt(num / den))
 
 
+def project_check(S, d):
+    """S = Q^H G Q for the Helmholtz operator G = d I + (real symmetric
+    stencil) and orthonormal Q satisfies S - S^H = 2 i Im(d) Q^H Q, so
+    ||S - S^H - 2 i Im(d) I||_F / ||S||_F measures S at full size."""
+    k = S.shape[0]
+    return float(np.linalg.norm(S - S.conj().T - 2j * d.imag * np.eye(k)) / np.linalg.norm(S))
+
+
+def adjoint_bench(backend):
+    """The adjoint workload: Y (4,194,304 x 128 ComplexF64, ghost panels) ->
+    Q R = Y -> S = Q^H G Q with G = diag(d) + the periodic tridiagonal
+    (2 + 0.1i, -1), d_i = 1 + 0.5 sin(2 pi i / N); loss
+    Re<dQ, Q> + Re<dS, S> for a seeded complex normal dQ and the Hermitian
+    part of a seeded complex normal dS. Times the forward pass (cholqr2 +
+    project, through their rrules) against the backward pass (both
+    pullbacks), and checks <Ybar, E> against a central difference of the
+    device forward pass along a ghost direction E."""
+    import funicular_hip as F
+    from funicular_hip.executor import readpanels, sweep, updatepanels, writepanels
+    N, k, w = 1 << 22, 128, 32
+    plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=1 << 30,
+                           device_resident=True)
+    d = 1.0 + 0.5 * np.sin(2 * np.pi * np.arange(N) / N)
+    G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, d, backend=backend)
+
+    def ghost_copy(seed):
+        src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=seed, w=w)
+        dst = F.PanelMatrix(N, k, plan=plan, w=w)
+        sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(dst))
+        return dst
+
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+    dS = (A + A.conj().T) / 2
+    dQ = ghost_copy(0xDEADBEEF + 4)
+
+    def forward(Y):
+        R, pb_qr = F.rrule_cholqr2(Y)
+        S, pb_pr = F.rrule_project(Y, G)
+        return R, S, pb_qr, pb_pr
+
+    def loss(Y, S):
+        return float(np.real(F.gram(dQ, Y).trace()) + np.real(np.vdot(dS, S)))
+
+    Y = ghost_copy(0xDEADBEEF + 3)
+    forward(Y)  # warm the pools
+    Y = ghost_copy(0xDEADBEEF + 3)
+    backend.sync_all()
+    t = time.perf_counter()
+    R, S, pb_qr, pb_pr = forward(Y)
+    backend.sync_all()
+    t_fwd = time.perf_counter() - t
+    t = time.perf_counter()
+    Qbar, dbar = pb_pr(dS)
+    sweep(lambda _, a, b: backend.axpby(b, a, 1.0, 1.0), readpanels(dQ), updatepanels(Qbar))
+    Ybar = pb_qr(Qbar, None)
+    backend.sync_all()
+    t_bwd = time.perf_counter() - t
+    log(f"adjoint: N={N} k={k}: forward {t_fwd:.3f} s, backward {t_bwd:.3f} s ({t_bwd / t_fwd:.2f}x)")
+
+    # Central difference along E of the device forward pass.
+    E = ghost_copy(0xDEADBEEF + 5)
+    h = 1e-6
+
+    def shifted(sign):
+        Yh = ghost_copy(0xDEADBEEF + 3)
+        sweep(lambda _, e, y: backend.axpby(y, e, sign * h, 1.0), readpanels(E), updatepanels(Yh))
+        F.cholqr2(Yh)
+        return loss(Yh, F.project(Yh, G))
+
+    fd = (shifted(1.0) - shifted(-1.0)) / (2 * h)
+    directional = float(np.real(F.gram(Ybar, E).trace()))
+    rel = abs(fd - directional) / max(abs(fd), 1e-300)
+    # And along a direction e of the diagonal parameters.
+    e = np.random.default_rng(5).standard_normal(N)
+
+    def shifted_d(sign):
+        Gh = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, d + sign * h * e, backend=backend)
+        return float(np.real(np.vdot(dS, F.project(Y, Gh))))
+
+    fd_d = (shifted_d(1.0) - shifted_d(-1.0)) / (2 * h)
+    rel_d = abs(fd_d - float(dbar @ e)) / max(abs(fd_d), 1e-300)
+    log(f"adjoint: <Ybar, E> vs central difference: rel {rel:.2e}; dS/dd along e: rel {rel_d:.2e}")
+    return {"workload": f"cholqr2 + project of a {N}x{k} ComplexF64 ghost basis, G = diag(d) + tridiagonal",
+            "forward_s": t_fwd, "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd,
+            "fd_rel_err_Y": rel, "fd_rel_err_d": rel_d, "fd_step": h}
+
+
 def kernels_only(rows, k, reps):
     """The per-launch shapes of the benchmark, without the pipeline around
     them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
@@ -281,6 +369,7 @@ def main():
     ap.add_argument("--w", type=int, default=32)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload (N = 2^22, k = 128)")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
     ap.add_argument("--streamed", action="store_true",
                     help="also run the host-resident (streamed over PCIe) cholqr2 and report overlap")
@@ -361,12 +450,22 @@ def main():
                      "frac": gram64["tflops"] / FP64_PEAK_TFLOPS,
                      "traffic": measured_traffic("gram_fp64") if (N, k, w) == (10077696, 512, 32) else None,
                      "avg_launch_ms": gram64["avg_ms"], "flops_per_launch": gram64["flops_per_launch"]},
-        "trsm": {"kernel": "trsm_diag_inv_kernel + trsm_runn_kernel (FP64)", "achieved": trsm64["tflops"],
+        "trsm": {"kernel": "trsm_block_inv_kernel + trsm_wide_kernel (FP64)", "achieved": trsm64["tflops"],
                  "peak": FP64_PEAK_TFLOPS, "frac": trsm64["tflops"] / FP64_PEAK_TFLOPS,
                  "avg_launch_ms": trsm64["avg_ms"]},
         "errors": {"fp64_orthogonality": orth64, "fp64_residual": res64},
         "panelmul_s": t_panelmul,
     }
+    # project on the orthonormal basis cholqr2 left in Y.
+    F.project(Y, G)
+    backend.sync_all()
+    t = time.perf_counter()
+    S = F.project(Y, G)
+    backend.sync_all()
+    t_project = time.perf_counter() - t
+    record["project"] = {"s": t_project, "reduction_tflops": 8.0 * N * k * k / t_project / 1e12,
+                         "antihermitian_check": project_check(S, d)}
+    log(f"bench: project {t_project:.3f} s, S - S^H check {record['project']['antihermitian_check']:.2e}")
     if not args.no_mixed:
         dtm, orthm, Rm, km, resm = run("mixed", args.steps, args.warmup)
         regenerate()
@@ -381,6 +480,9 @@ def main():
                            "gram_fp32_tflops": summarize(km[("gram", True)], gram_flops)["tflops"],
                            "trsm_fp32_tflops": summarize(km[("trsm", True)], trsm_flops)["tflops"],
                            "R_rel_diff_vs_fp64": float(np.linalg.norm(Rm - R64) / np.linalg.norm(R64))}
+    if not args.no_adjoint:
+        del S
+        record["adjoint"] = adjoint_bench(backend)
     if args.streamed:
         del Y, omega, plan
         import gc

@@ -59,6 +59,16 @@ struct GramShape<32> {
   static constexpr int RESIDENT = 1024;
 };
 
+// Rows staged per LDS pass for the FP32 products of the 64 tile (a tuning
+// knob; 32 measured slower than 16 at 629856 x 512).
+#ifndef FHIP_GRAM_KC_FP32
+#define FHIP_GRAM_KC_FP32 16
+#endif
+template <typename TC, int TILE>
+constexpr int gram_kc() {
+  return (sizeof(TC) == 4 && TILE == 64) ? FHIP_GRAM_KC_FP32 : GramShape<TILE>::KC;
+}
+
 constexpr int GRAM_THREADS = 256;
 constexpr int GRAM_MIN_CHUNK_ROWS = 256;
 
@@ -94,7 +104,9 @@ static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm, bool fp3
   if (p.tile == 128) {
     kc = GramShape<128>::KC; p.ksplit = GramShape<128>::KSPLIT; resident = GramShape<128>::RESIDENT;
   } else if (p.tile == 64) {
-    kc = GramShape<64>::KC; p.ksplit = GramShape<64>::KSPLIT; resident = GramShape<64>::RESIDENT;
+    kc = fp32 ? gram_kc<float, 64>() : gram_kc<double, 64>();
+    p.ksplit = GramShape<64>::KSPLIT;
+    resident = GramShape<64>::RESIDENT;
   } else {
     kc = GramShape<32>::KC; p.ksplit = GramShape<32>::KSPLIT; resident = GramShape<32>::RESIDENT;
   }
@@ -129,7 +141,7 @@ template <typename TIn, typename TC, bool CPLX, int TILE>
 __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void gram_partial_kernel(
     const TIn *__restrict__ A, int64_t lda, const TIn *__restrict__ B, int64_t ldb, int64_t m, int ka,
     int kb, int64_t chunk_rows, int ntiles, int ntm, int herm, TC *__restrict__ partial) {
-  constexpr int KC = GramShape<TILE>::KC;
+  constexpr int KC = gram_kc<TC, TILE>();
   constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
   constexpr int MB = GramShape<TILE>::MB;
   constexpr int NB = GramShape<TILE>::NB;
@@ -244,18 +256,26 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
     }
 }
 
-// Adds the partial slots in slot order and applies alpha and beta.
+// Adds the partial slots and applies alpha and beta. A workgroup takes
+// 256 / rg elements; the rg thread groups of an element sum the slots
+// s = g, g + rg, ... in order, and an LDS tree adds the groups in a fixed
+// pairing. rg is a function of the slot count alone, so the bits depend on
+// (m, ka, kb, compute eltype) as the partial sums do.
 template <typename TC, typename TOut, bool CPLX, int TILE>
 __global__ __launch_bounds__(256) void gram_reduce_kernel(const TC *__restrict__ partial, int nslots,
                                                           int ntiles, int ntm, int herm,
                                                           TOut *__restrict__ C, int64_t ldc, int ka,
                                                           int kb, double al_re, double al_im,
-                                                          double be_re, double be_im, int beta_zero) {
+                                                          double be_re, double be_im, int beta_zero, int rg) {
   constexpr int NP = CPLX ? 2 : 1;
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= (int64_t)ntiles * TILE * TILE) return;
-  const int tile = (int)(e / (TILE * TILE));
-  const int within = (int)(e % (TILE * TILE));
+  __shared__ double red_re[256], red_im[256];
+  const int epb = 256 / rg;
+  const int el = threadIdx.x % epb;
+  const int g = threadIdx.x / epb;
+  const int64_t e = (int64_t)blockIdx.x * epb + el;
+  const bool in_grid = e < (int64_t)ntiles * TILE * TILE;
+  const int tile = in_grid ? (int)(e / (TILE * TILE)) : 0;
+  const int within = in_grid ? (int)(e % (TILE * TILE)) : 0;
   const int jl = within / TILE;
   const int il = within % TILE;
   int ti, tj;
@@ -267,19 +287,35 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const TC *__restrict__
   }
   const int i = ti * TILE + il;
   const int j = tj * TILE + jl;
-  if (i >= ka || j >= kb) return;
+  // Every thread reaches the barriers below; only live ones load and store.
+  const bool live = in_grid && i < ka && j < kb;
   // Below the diagonal of a diagonal tile, take the mirrored upper element so
   // the result is exactly Hermitian.
   const bool lower = herm && ti == tj && il > jl;
   const int src = lower ? (il * TILE + jl) : within;
   double sr = 0.0, si = 0.0;
   const int64_t stride = (int64_t)ntiles * TILE * TILE * NP;
-  const TC *p = partial + ((int64_t)tile * TILE * TILE + src) * NP;
-  for (int s = 0; s < nslots; ++s) {
-    sr += (double)p[0];
-    if (CPLX) si += (double)p[1];
-    p += stride;
+  const TC *p = partial + ((int64_t)g * ntiles * TILE * TILE + (int64_t)tile * TILE * TILE + src) * NP;
+  if (live) {
+    for (int s = g; s < nslots; s += rg) {
+      sr += (double)p[0];
+      if (CPLX) si += (double)p[1];
+      p += stride * rg;
+    }
   }
+  red_re[threadIdx.x] = sr;
+  red_im[threadIdx.x] = si;
+  for (int h = rg / 2; h >= 1; h /= 2) {
+    __syncthreads();
+    if (g < h) {
+      red_re[threadIdx.x] += red_re[threadIdx.x + h * epb];
+      red_im[threadIdx.x] += red_im[threadIdx.x + h * epb];
+    }
+  }
+  __syncthreads();
+  if (g != 0 || !live) return;
+  sr = red_re[threadIdx.x];
+  si = red_im[threadIdx.x];
   if (lower) si = -si;
   // The diagonal of A^H A is real.
   if (herm && i == j) si = 0.0;
@@ -300,6 +336,14 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const TC *__restrict__
   if (herm && ti < tj && j < ka && i < kb) store((int64_t)j + (int64_t)i * ldc, sr, -si);
 }
 
+// Slot groups of the reduce: about 32 slots per thread, a power of two
+// between 1 and 256.
+static int reduce_groups(int nslots) {
+  int rg = 1;
+  while (rg < 256 && rg * 64 <= nslots) rg *= 2;
+  return rg;
+}
+
 template <typename TIn, typename TC, typename TOut, bool CPLX, int TILE>
 static int launch_gram(const GramPlan &p, TOut *C, int64_t ldc, const TIn *A, int64_t lda, const TIn *B,
                        int64_t ldb, int64_t m, int ka, int kb, bool herm, double ar, double ai,
@@ -312,11 +356,12 @@ static int launch_gram(const GramPlan &p, TOut *C, int64_t ldc, const TIn *A, in
   int st = check_launch("gram_partial_kernel");
   if (st) return st;
   const int64_t elems = (int64_t)p.ntiles * TILE * TILE;
-  const int rgrid = (int)cdiv64(elems, 256);
+  const int rg = reduce_groups(p.nslots);
+  const int rgrid = (int)cdiv64(elems, 256 / rg);
   const int beta_zero = (br == 0.0 && bi == 0.0) ? 1 : 0;
   hipLaunchKernelGGL((gram_reduce_kernel<TC, TOut, CPLX, TILE>), dim3(rgrid), dim3(256), 0, stream,
                      partial, p.nslots, p.ntiles, p.ntm, herm ? 1 : 0, C, ldc, ka, kb, ar, ai, br, bi,
-                     beta_zero);
+                     beta_zero, rg);
   return check_launch("gram_reduce_kernel");
 }
 
