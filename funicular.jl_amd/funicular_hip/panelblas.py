@@ -292,27 +292,35 @@ def cholesky_upper_fp32(G, N, k, shift="adaptive"):
     return R, s
 
 
-def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive"):
+def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp64"):
     """The mixed first pass: FP32 matrix-core Gram with FP64 sums across row
-    chunks, FP32 Cholesky (shifted when needed), FP32 triangular solve on the
-    FP64 data. Returns (R, shift)."""
+    chunks and an FP32 Cholesky (shifted when needed), then the triangular
+    solve on the FP64 data. solve="fp64" solves in FP64 with the FP32-derived
+    factor, so Q R reproduces Y to FP64 rounding; solve="fp32" runs the solve's
+    products on the FP32 matrix cores, which leaves ||Y - Q R|| / ||Y|| near
+    u32 kappa(Y). Returns (R, shift)."""
     if Y.dtype != np.dtype(np.complex128):
         raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
+    if solve not in ("fp64", "fp32"):
+        raise ArgumentError(f"solve must be 'fp64' or 'fp32', got {solve!r}")
     G = gram(Y, nbuffers=nbuffers, compute=np.complex64)
     R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
     R = R32.astype(Y.dtype)
-    rdiv_rows(Y, R, nbuffers=nbuffers, compute=np.complex64)
+    rdiv_rows(Y, R, nbuffers=nbuffers, compute=np.complex64 if solve == "fp32" else None)
     return R, s
 
 
-def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive"):
-    """cholqr2 with the first pass in mixed precision (FP32 Gram and solve) as
-    a preconditioner and the second pass in FP64. A third FP64 pass runs when
-    the second one's Cholesky needed the reference's shift."""
-    R1, _ = cholqr_pass_mixed(Y, nbuffers, shift)
+def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp64"):
+    """cholqr2 with a mixed-precision first pass (FP32 Gram and Cholesky, the
+    solve in the precision `solve` names) as a preconditioner and the second
+    pass in FP64. A third FP64 pass runs when either earlier Cholesky needed a
+    shift, as shifted CholeskyQR3 prescribes: a shifted first pass leaves Q1
+    with a condition number well above 1, which one FP64 pass does not bring
+    to FP64 orthogonality."""
+    R1, s = cholqr_pass_mixed(Y, nbuffers, shift, solve)
     R2, shifted = cholqr_pass(Y, nbuffers)
     R = R2 @ R1
-    if shifted:
+    if shifted or s > 0.0:
         R3, _ = cholqr_pass(Y, nbuffers)
         R = R3 @ R
     return R

@@ -1,0 +1,83 @@
+"""cholqr2 with the mixed-precision first pass on Y = U diag(sigma) V^H with
+log-spaced singular values (the mixed-precision workload at test size):
+orthogonality at FP64 level, R against the FP64 cholqr2's, and Q R against
+Y, with the solve in FP64 and in FP32."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import funicular_hip as F
+from funicular_hip.panelblas import cholesky_upper_fp32, cholqr2_mixed
+from oracle import panelblas as OB
+
+pytestmark = pytest.mark.gpu
+
+
+def graded(rng, N, k, decades):
+    U = scipy.linalg.qr(rng.standard_normal((N, k)) + 1j * rng.standard_normal((N, k)), mode="economic")[0]
+    V = scipy.linalg.qr(rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k)))[0]
+    sigma = 10.0 ** (-decades * np.arange(k) / (k - 1))
+    return np.asfortranarray((U * sigma) @ V.conj().T)
+
+
+def make_plan(backend):
+    return F.ResidencyPlan(backend=backend, device_budget=256 * 2 ** 20, host_budget=256 * 2 ** 20)
+
+
+@pytest.mark.parametrize("decades", (2, 4))
+@pytest.mark.parametrize("solve", ("fp64", "fp32"))
+@pytest.mark.parametrize("w", (16, 7))
+def test_cholqr2_mixed_on_graded_spectrum(backend, decades, solve, w):
+    rng = np.random.default_rng(60 + decades)
+    N, k = 20000, 64
+    Y0 = graded(rng, N, k, decades)
+    plan = make_plan(backend)
+    Y = F.PanelMatrix.from_array(Y0, plan=plan, w=w)
+    R = cholqr2_mixed(Y, solve=solve)
+    Q = F.to_matrix(Y)
+    assert np.allclose(R, np.triu(R))
+    assert np.all(np.diag(R).real > 0)
+    assert OB.orthogonality_error(Q) <= 1e-13
+    resid = np.linalg.norm(Q @ R - Y0) / np.linalg.norm(Y0)
+    if solve == "fp64":
+        assert resid <= 1e-13
+    else:
+        # The FP32 solve leaves an error of order u32 kappa(Y) in Q R.
+        assert resid <= 1e-7 * 10.0 ** decades
+    Qo, Ro = OB.cholqr2(Y0, w)
+    # R agrees with the FP64 factor to the conditioning of the problem.
+    assert np.linalg.norm(R - Ro) / np.linalg.norm(Ro) <= (1e-14 if solve == "fp64" else 1e-6) * 10.0 ** decades
+
+
+def test_cholqr2_mixed_matches_fp64_on_well_conditioned(backend):
+    rng = np.random.default_rng(61)
+    N, k, w = 30000, 40, 8
+    Y0 = np.asfortranarray(rng.standard_normal((N, k)) + 1j * rng.standard_normal((N, k)))
+    plan = make_plan(backend)
+    Y = F.PanelMatrix.from_array(Y0, plan=plan, w=w)
+    R = cholqr2_mixed(Y)
+    Q = F.to_matrix(Y)
+    Z = F.PanelMatrix.from_array(Y0, plan=plan, w=w)
+    R64 = F.cholqr2(Z)
+    assert OB.isapprox(R, R64, 1e-13)
+    assert OB.isapprox(Q, F.to_matrix(Z), 1e-13)
+
+
+def test_fp32_cholesky_shift_policy(backend):
+    # Host-side rule, exercised here beside its kernels: "adaptive" shifts only
+    # when the unshifted FP32 factorization fails; "fukaya" always shifts.
+    rng = np.random.default_rng(62)
+    A = rng.standard_normal((500, 20)) + 1j * rng.standard_normal((500, 20))
+    G = A.conj().T @ A
+    R, s = cholesky_upper_fp32(G, 500, 20)
+    assert s == 0.0 and R.dtype == np.complex64
+    R, s = cholesky_upper_fp32(G, 500, 20, shift="fukaya")
+    lam = np.linalg.eigvalsh(G)[-1]
+    assert s == pytest.approx(11.0 * (500 * 20 + 20 * 21) * 2.0 ** -24 * lam, rel=1e-12)
+    # A singular Gram: the unshifted factorization fails and the shift takes over.
+    B = A.copy()
+    B[:, 3] = B[:, 2]
+    R, s = cholesky_upper_fp32(B.conj().T @ B, 500, 20)
+    assert s > 0.0
+    with pytest.raises(F.ArgumentError):
+        cholqr2_mixed(F.PanelMatrix.from_array(np.asfortranarray(A), plan=make_plan(backend), w=5), solve="fp16")
