@@ -17,6 +17,7 @@ from .adjoint import rrule_cholqr2, rrule_project  # noqa: E402
 from .executor import (PanelSteps, RowSteps, foreachpanel, readpanels, rowtraversal, sweep,  # noqa: E402
                        updatepanels, writepanels)
 from .ghosts import GhostPanels  # noqa: E402
+from .io import disk_reads, disk_writes, load, save  # noqa: E402
 from .operator import (DenseOperator, Helmholtz3D, Tridiagonal1D, check_operator, ishermitian_op,  # noqa: E402
                        panel_capable, workspace_bytes)
 from .panelblas import (cholesky_upper, cholqr2, cholqr2_mixed, gram, panelmul, project, rdiv_rows,  # noqa: E402
@@ -29,5 +30,5 @@ from .plan import (ResidencyPlan, choose_panel_width, device_bytes_allocated, re
 __all__ = [
     "GhostPanels", "PanelMatrix", "ResidencyPlan", "check_operator", "cholqr2", "copycols", "foreachpanel",
     "gram", "npanels", "panelmul", "panelrange", "panelwidth", "project", "rightmul", "hip_backend",
-    "rrule_cholqr2", "rrule_project",
+    "rrule_cholqr2", "rrule_project", "save", "load", "disk_reads", "disk_writes",
 ]

@@ -129,20 +129,29 @@ class PanelMatrix:
                 self.panels.append(p)
             return
         S = host_storage_eltype(plan, T)
-        disk_storage_eltype(plan, T)
+        D = disk_storage_eltype(plan, T)
+        # A plan with a scratch_dir gives the matrix a file of its own, and
+        # panels spill to it when the host tier runs out (src/panels.jl:57-61).
+        if plan.scratch_dir is not None:
+            from .io import DiskHome, scratch_store
+            self.store = scratch_store(plan, self.N, self.k, width, D, T)
         # Every panel takes a block sized for the nominal width, ragged one
         # included, so a released block fits the next panel (src/panels.jl:64-67).
         blockbytes = _plan.panel_bytes(self.N, width, S)
         try:
             for j in range(npan):
                 c0, c1 = j * width, min((j + 1) * width, self.k)
-                p = Panel(None, HOST, None, c1 - c0)
+                home = None if self.store is None else DiskHome(self.store, range(c0, c1))
+                p = Panel(None, HOST, home, c1 - c0)
                 self.panels.append(p)
                 p.data = _plan.alloc_host_storage(plan, S, (self.N, c1 - c0), blockbytes)
                 _plan.register_resident(plan, p)
         except BaseException:
             for p in self.panels:
                 release(plan, p)
+            if self.store is not None:
+                from .io import close_store
+                close_store(self.store)
             raise
 
     @classmethod
