@@ -271,6 +271,50 @@ def adjoint_bench(backend):
             "fd_rel_err_Y": rel, "fd_rel_err_d": rel_d, "fd_step": h}
 
 
+def disk_bench(backend, scratch):
+    """The disk tier under a panel sweep: Y = G Omega for a 4,194,304 x 256
+    ComplexF64 basis in panels of 32 (8 panels, 17.2 GB) with a host budget
+    of four panels, so every output panel goes to the scratch file and a
+    read sweep (one 32 x 32 Gram per panel) brings each back; the same two
+    sweeps with the whole basis in host memory give the cost of the tier.
+    The scratch file sits on the box's local disk, behind the page cache."""
+    import funicular_hip as F
+    from funicular_hip.executor import readpanels, sweep
+    N, k, w = 1 << 22, 256, 32
+    G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, backend=backend)
+    panel = N * w * 16
+    out = {"workload": f"panelmul + read sweep of a {N}x{k} ComplexF64 basis, panels of {w}",
+           "panel_GB": panel / 1e9}
+    for label, budget, sdir in (("disk", 4 * panel + (1 << 20), scratch), ("host", 9 * panel, None)):
+        plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=budget,
+                               scratch_dir=sdir)
+        omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+        Y = F.PanelMatrix(N, k, plan=plan, w=w)
+        C = backend.alloc_device(np.complex128, (w, w))
+        backend.sync_all()
+        t = time.perf_counter()
+        F.panelmul(Y, G, omega)
+        backend.sync_all()
+        t_mul = time.perf_counter() - t
+        reads0 = F.disk_reads(Y)
+        t = time.perf_counter()
+        sweep(lambda _, b: backend.gram_accumulate(C, b, b, 1.0, 0.0), readpanels(Y))
+        backend.sync_all()
+        t_read = time.perf_counter() - t
+        rec = {"panelmul_s": t_mul, "read_sweep_s": t_read}
+        if sdir is not None:
+            rec.update({"disk_writes": F.disk_writes(Y), "disk_reads_in_read_sweep": F.disk_reads(Y) - reads0,
+                        "write_GBps": F.disk_writes(Y) * panel / t_mul / 1e9,
+                        "read_GBps": (F.disk_reads(Y) - reads0) * panel / t_read / 1e9})
+        out[label] = rec
+        log(f"disk tier ({label}): panelmul {t_mul:.2f} s, read sweep {t_read:.2f} s, {rec}")
+        F.free(Y)
+        del plan, omega, Y, C
+        import gc
+        gc.collect()
+    return out
+
+
 def kernels_only(rows, k, reps):
     """The per-launch shapes of the benchmark, without the pipeline around
     them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
@@ -370,6 +414,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload (N = 2^22, k = 128)")
+    ap.add_argument("--disk", action="store_true", help="also measure the disk tier (17 GB scratch file)")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
     ap.add_argument("--streamed", action="store_true",
                     help="also run the host-resident (streamed over PCIe) cholqr2 and report overlap")
@@ -488,6 +533,10 @@ def main():
     if not args.no_adjoint:
         del S
         record["adjoint"] = adjoint_bench(backend)
+    if args.disk:
+        import tempfile
+        with tempfile.TemporaryDirectory(prefix="funicular-disk-") as d:
+            record["disk_tier"] = disk_bench(backend, d)
     if args.streamed:
         del Y, omega, plan
         import gc

@@ -13,11 +13,11 @@ attributes in a raw form:
     bytes [4096, ...)   the matrix, column major, in the storage eltype
 
 A panel's columns are contiguous in column-major order, so one panel is one
-contiguous region at 4096 + N * first_column * itemsize: one pread or pwrite
-per panel, the chunk being the unit of I/O exactly as in the HDF5 layout. The
-reads land directly in the panel's page-locked host block, on the sweep's
-producer thread, serialized by DISK_LOCK as the reference serializes its HDF5
-calls.
+contiguous region at 4096 + N * first_column * itemsize, and the chunk is the
+unit of I/O exactly as in the HDF5 layout. A panel's region moves as parallel
+64 MiB preads or pwrites straight into (out of) the panel's page-locked host
+block, on the sweep's producer thread, and DISK_LOCK serializes the panels as
+the reference serializes its HDF5 calls.
 """
 import itertools
 import json
@@ -158,16 +158,42 @@ def _pwrite_all(fd, mv, offset):
         done += n
 
 
+def _pread_all(fd, mv, offset):
+    done = 0
+    while done < len(mv):
+        n = os.preadv(fd, [mv[done:]], offset + done)
+        if n <= 0:
+            raise OSError("a panel store ended before the region read from it")
+        done += n
+
+
+# A panel is gigabytes at the configurations' sizes, and one thread copying
+# it through the page cache runs at a few GB/s. The region is cut into pieces
+# that a small pool moves in parallel (the system calls release the GIL).
+IO_PIECE_BYTES = 64 << 20
+IO_THREADS = 8
+_io_pool = None
+
+
+def _parallel(fn, fd, mv, offset):
+    global _io_pool
+    if len(mv) <= IO_PIECE_BYTES:
+        fn(fd, mv, offset)
+        return
+    if _io_pool is None:
+        import concurrent.futures
+        _io_pool = concurrent.futures.ThreadPoolExecutor(max_workers=IO_THREADS, thread_name_prefix="panel-io")
+    pieces = [_io_pool.submit(fn, fd, mv[a:a + IO_PIECE_BYTES], offset + a)
+              for a in range(0, len(mv), IO_PIECE_BYTES)]
+    for f in pieces:
+        f.result()
+
+
 def read_panel_region(handle, dst, cols):
     """Reads columns `cols` of the stored matrix into the host array dst, of
     the store's eltype and exactly their shape (src/io.jl:34-40)."""
     mv, offset = _region(handle, cols, dst)
-    done = 0
-    while done < len(mv):
-        n = os.preadv(handle.fd, [mv[done:]], offset + done)
-        if n <= 0:
-            raise OSError(f"the panel store at {handle.path!r} ended before columns {cols}")
-        done += n
+    _parallel(_pread_all, handle.fd, mv, offset)
     return dst
 
 
@@ -176,7 +202,7 @@ def write_panel_region(handle, src, cols):
     if not handle.writable:
         raise ArgumentError(f"the store at {handle.path!r} is open for reading only")
     mv, offset = _region(handle, cols, np.asfortranarray(src))
-    _pwrite_all(handle.fd, mv, offset)
+    _parallel(_pwrite_all, handle.fd, mv, offset)
     return src
 
 
