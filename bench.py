@@ -271,6 +271,67 @@ def adjoint_bench(backend):
             "fd_rel_err_Y": rel, "fd_rel_err_d": rel_d, "fd_step": h}
 
 
+def mixed_bench(backend, steps):
+    """The mixed-precision workload: Y = U diag(sigma) V^H with U the Q of
+    cholqr2 of an 8,388,608 x 256 ComplexF64 ghost basis (seed 0xdeadbeef),
+    V the unitary Q of a seeded 256 x 256 complex normal, and sigma
+    log-spaced from 1 to 1e-2 and, in a second case, to 1e-4. For each case,
+    FP64-only cholqr2 against the mixed first pass, with orthogonality and
+    the residual ||Y - Q R|| / ||Y|| measured on the device."""
+    import funicular_hip as F
+    from funicular_hip.executor import readpanels, sweep, writepanels
+    from funicular_hip.panelblas import cholqr2_mixed
+    N, k, w = 8388608, 256, 16
+    plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=1 << 30,
+                           device_resident=True)
+    U = F.PanelMatrix(N, k, plan=plan, w=w)
+    src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(U))
+    F.cholqr2(U)
+    rng = np.random.default_rng(0xDEADBEEF + 1)
+    V = np.linalg.qr(rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k)))[0]
+    Y0 = F.PanelMatrix(N, k, plan=plan, w=w)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    out = {"workload": f"cholqr2 of Y = U diag(sigma) V^H, {N}x{k} ComplexF64, panels of {w}"}
+
+    def copy(dst, src_):
+        sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src_), writepanels(dst))
+
+    def resid(R):
+        # ||Y0 - Q R||_F / ||Y0||_F with Q in Y, through one more product.
+        Z = F.PanelMatrix(N, k, plan=plan, w=w)
+        F.rightmul(Z, Y, R)
+        sweep(lambda _, a, b: backend.axpby(b, a, -1.0, 1.0), readpanels(Y0), F.updatepanels(Z))
+        num = float(np.real(np.trace(F.gram(Z))))
+        F.free(Z)
+        return float(np.sqrt(num / np.real(np.trace(F.gram(Y0)))))
+
+    for decades in (2, 4):
+        sigma = 10.0 ** (-decades * np.arange(k) / (k - 1))
+        F.rightmul(Y0, U, sigma[:, None] * V.conj().T)
+        case = {}
+        for name, fn in (("fp64", F.cholqr2), ("mixed", cholqr2_mixed)):
+            copy(Y, Y0)
+            R = fn(Y)  # accuracy from the first run
+            orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+            res = resid(R)
+            times = []
+            for _ in range(steps):
+                copy(Y, Y0)
+                backend.sync_all()
+                t = time.perf_counter()
+                fn(Y)
+                backend.sync_all()
+                times.append(time.perf_counter() - t)
+            case[name] = {"s": min(times), "orthogonality": orth, "residual": res}
+        case["speedup"] = case["fp64"]["s"] / case["mixed"]["s"]
+        out[f"sigma_to_1e-{decades}"] = case
+        log(f"mixed workload, sigma to 1e-{decades}: {case}")
+    for pm in (U, Y0, Y):
+        F.free(pm)
+    return out
+
+
 def disk_bench(backend, scratch):
     """The disk tier under a panel sweep: Y = G Omega for a 4,194,304 x 256
     ComplexF64 basis in panels of 32 (8 panels, 17.2 GB) with a host budget
@@ -530,6 +591,8 @@ def main():
                            "fp32_solve": {"rows_per_s": N * args.steps / dts, "speedup_vs_fp64": dt64 / dts,
                                           "orthogonality": orths, "residual": ress,
                                           "trsm_fp32_tflops": summarize(ks[("trsm", True)], trsm_flops)["tflops"]}}
+    if not args.no_mixed:
+        record["mixed_graded"] = mixed_bench(backend, args.steps)
     if not args.no_adjoint:
         del S
         record["adjoint"] = adjoint_bench(backend)
