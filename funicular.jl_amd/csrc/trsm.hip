@@ -4,61 +4,74 @@
 // for the QR adjoint.
 //
 // Design (gfx950). Rows are independent, so a workgroup owns TW_ROWS rows and
-// walks the columns in blocks of TW_CB, left looking:
-//     X_J = (Y_J - sum_{I<J} X_I R_IJ) * inv(R_JJ).
-// Wide blocks keep HBM traffic down: the update re-reads the workgroup's
-// solved columns once per block, so per row it reads sum_J TW_CB J columns,
-// plus k for the right-hand side and k for the diagonal multiply (at k = 512:
-// 2.8 k for 64-wide blocks, 8.5 k for 32-wide ones). The inverses of the
-// diagonal blocks are formed once per call, in double, by
-// trsm_block_inv_kernel (a right-looking sweep over the rows of the inverse,
-// one barrier per row). The update and the multiply by the
-// diagonal inverse both run as three-product (3M) complex matrix-core steps
-// over LDS-staged 16-column slices, the next slice's loads in flight during
-// the current slice's products. Between the two, the block's right-hand side
-// makes one round trip through the workgroup's own rows of Y, which are then
-// hot in L2.
+// walks the columns in blocks of TW_CB, left looking. With the diagonal
+// inverses folded into the factor,
+//     M_JJ = inv(R_JJ),   M_IJ = -R_IJ inv(R_JJ)  (I < J),
+// block J of the solution is one product over the columns already at hand,
+//     X_J = [Y_J, X_0, ..., X_{J-1}] * [M_JJ; M_0J; ...; M_{J-1,J}],
+// so the whole solve is a chain of sums that never leaves the matrix cores'
+// pipeline: the workgroup streams 16-column slices of its rows and of M
+// through LDS, the next slice's loads in flight during the current slice's
+// three-product (3M) complex steps, across block boundaries too. Each block
+// reads its unsolved columns first and the solved ones after, so a slice in
+// flight never races the block being written: the newest solved block is
+// read several barriers after its stores. M is formed once per call in
+// double: trsm_block_inv_kernel inverts the diagonal blocks (a right-looking
+// sweep over the rows of the inverse, one barrier per row) and
+// trsm_offdiag_kernel multiplies them into the blocks above. Per row, the
+// workgroup reads sum_J TW_CB (J + 1) columns and writes k (2.25 k reads at
+// k = 512 with 64-column blocks).
+//
+// Numerically this is blocked substitution with inverted diagonal blocks: the
+// error of a block is governed by kappa of its 64 x 64 diagonal block, not by
+// kappa(R).
 #include "fhip_internal.hpp"
 
 #include <type_traits>
 
 namespace fhip {
 
-// Column-block width; a workgroup owns 4096 / TW_CB rows, so that its four
-// waves each hold a 32 x 32 tile of the rows x TW_CB block.
 #ifndef FHIP_TRSM_CB
 #define FHIP_TRSM_CB 64
 #endif
-constexpr int TW_THREADS = 256;  // 4 waves
-constexpr int TW_CB = FHIP_TRSM_CB;
-constexpr int TW_ROWS = 4096 / TW_CB;
-// Inner dimension per LDS slice (a tuning knob per compute precision; 32
-// for FP32 measured slower than 16 at 629856 x 512).
+// Inner dimension per LDS slice for FP32 compute (a tuning knob; 32 measured
+// slower than 16 at 629856 x 512).
 #ifndef FHIP_TRSM_KC_FP32
 #define FHIP_TRSM_KC_FP32 16
 #endif
+constexpr int TW_THREADS = 256;  // 4 waves, each a 32 x 32 tile of the block
+constexpr int TW_CB = FHIP_TRSM_CB;
+constexpr int TW_ROWS = 4096 / TW_CB;
+constexpr int TW_LDA = TW_ROWS + 4;
+constexpr int TW_LDB = TW_CB + 4;
+constexpr int TI_THREADS = 8 * TW_CB;  // block inverse: 8 row groups x TW_CB columns
+constexpr int TI_RPT = TW_CB / 8;      // rows of the inverse per thread
+constexpr int TI_CHUNK = 16;           // columns of R staged in LDS at a time
+static_assert(TW_CB == 32 || TW_CB == 64 || TW_CB == 128, "column block of 32, 64 or 128");
+
+#ifndef FHIP_TRSM_KC_FP64
+#define FHIP_TRSM_KC_FP64 16
+#endif
 template <typename T>
 struct TrsmSlice {
-  static constexpr int KC = sizeof(T) == 4 ? FHIP_TRSM_KC_FP32 : 16;
+  static constexpr int KC = sizeof(T) == 4 ? FHIP_TRSM_KC_FP32 : FHIP_TRSM_KC_FP64;
   static constexpr int APT = KC * TW_ROWS / TW_THREADS;  // A elements per thread per slice
   static constexpr int BPT = KC * TW_CB / TW_THREADS;    // B elements per thread per slice
 };
-constexpr int TW_LDA = TW_ROWS + 4;
-constexpr int TW_LDB = TW_CB + 4;
-constexpr int TI_THREADS = 8 * TW_CB;                 // block inverse: 8 row groups x TW_CB columns
-constexpr int TI_RPT = TW_CB / 8;                     // rows of the inverse per thread
-constexpr int TI_CHUNK = 16;                          // columns of R staged in LDS at a time
-static_assert(TW_CB == 32 || TW_CB == 64 || TW_CB == 128, "column block of 32, 64 or 128");
+
+// M is kp x kp, kp = k rounded up to TW_CB, column major.
+inline int64_t trsm_kp(int64_t k) { return cdiv64(k, TW_CB) * TW_CB; }
 
 // Inverse of each TW_CB x TW_CB diagonal block of R (the last one cut to
-// what is left of k, zero padded), column major, one block after another,
-// computed in double. W = inv(R_JJ) solves R_JJ W = I from the bottom row
-// up: once row l of W is known, its multiple R[i, l] W[l, :] is taken off
-// every row i < l. Thread (g, j) keeps column j of rows g, g + 8, ... in
-// registers; R comes through LDS sixteen columns at a time.
-template <typename TS, typename T, bool CPLX>
+// what is left of k, zero padded), written into the diagonal blocks of M,
+// conjugate transposed when CT. Computed in double. W = inv(R_JJ) solves
+// R_JJ W = I from the bottom row up: once row l of W is known, its multiple
+// R[i, l] W[l, :] is taken off every row i < l. Thread (g, j) keeps column j
+// of rows g, g + 8, ... in registers; R comes through LDS sixteen columns at
+// a time.
+template <typename TS, typename T, bool CPLX, bool CT>
 __global__ __launch_bounds__(TI_THREADS) void trsm_block_inv_kernel(const TS *__restrict__ R, int64_t ldr, int k,
-                                                                     T *__restrict__ Dinv) {
+                                                                     T *__restrict__ M, int64_t kp) {
   constexpr int NP = CPLX ? 2 : 1;
   __shared__ double rc_re[TI_CHUNK][TW_CB], rc_im[TI_CHUNK][TW_CB];
   __shared__ double row_re[2][TW_CB], row_im[2][TW_CB];
@@ -127,24 +140,95 @@ __global__ __launch_bounds__(TI_THREADS) void trsm_block_inv_kernel(const TS *__
       }
     }
   }
-  T *Z = Dinv + (int64_t)J * TW_CB * TW_CB * NP;
 #pragma unroll
   for (int q = 0; q < TI_RPT; ++q) {
     const int i = g + 8 * q;
     const bool keep = i <= j && j < nb;
-    Z[(i + (int64_t)j * TW_CB) * NP] = keep ? (T)br[q] : T(0);
-    if (CPLX) Z[(i + (int64_t)j * TW_CB) * NP + 1] = keep ? (T)bi[q] : T(0);
+    // Element (i, j) of inv(R_JJ), or element (j, i) of its conjugate transpose.
+    const int64_t at = CT ? (int64_t)(c0 + j) + (int64_t)(c0 + i) * kp : (int64_t)(c0 + i) + (int64_t)(c0 + j) * kp;
+    M[at * NP] = keep ? (T)br[q] : T(0);
+    if (CPLX) M[at * NP + 1] = keep ? (T)(CT ? -bi[q] : bi[q]) : T(0);
   }
 }
 
-// The solve. CT = false solves X R = Y (left to right over column blocks);
-// CT = true solves X R^H = Y, whose factor R^H is lower triangular, right to
-// left. Each wave computes a 32 x 32 piece of the workgroup's
-// TW_ROWS x TW_CB block as 2 x 2 matrix-core tiles.
+// The off-diagonal blocks of M: M_IJ = -F_IJ M_JJ with F = R (I < J) or
+// F = R^H (I > J, when CT), one workgroup per block, in double, the factors
+// staged in LDS 32 inner columns at a time.
 template <typename TS, typename T, bool CPLX, bool CT>
-__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void trsm_wide_kernel(
-    TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const TS *__restrict__ R, int64_t ldr,
-    const T *__restrict__ Dinv) {
+__global__ __launch_bounds__(256) void trsm_offdiag_kernel(const TS *__restrict__ R, int64_t ldr, int k,
+                                                           T *__restrict__ M, int64_t kp) {
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int B = TW_CB;
+  constexpr int LC = 32 < B ? 32 : B;             // inner columns per pass
+  constexpr int ROWS = B / (256 / B);             // output rows per thread
+  __shared__ double f_re[B][LC + 1], f_im[B][LC + 1];  // F_IJ[i][l0 + l]
+  __shared__ double d_re[LC][B + 1], d_im[LC][B + 1];  // M_JJ[l0 + l][j]
+  // Pair p -> (I, J): I < J, J = 1, 2, ..., pairs of one J consecutive.
+  int p = blockIdx.x, J = 1;
+  while (p >= J) {
+    p -= J;
+    ++J;
+  }
+  const int nbk = (k + B - 1) / B;
+  int I = p;
+  if (CT) {
+    // Mirror: I > J.
+    const int Jm = nbk - 1 - J, Im = nbk - 1 - I;
+    I = Im;
+    J = Jm;
+  }
+  const int cI = I * B, cJ = J * B;
+  const int t = threadIdx.x;
+  const int j = t % B;
+  const int g = t / B;
+  double sr[ROWS], si[ROWS];
+  for (int q = 0; q < ROWS; ++q) sr[q] = si[q] = 0.0;
+  for (int l0 = 0; l0 < B; l0 += LC) {
+    __syncthreads();
+    for (int e = t; e < B * LC; e += 256) {
+      const int i = e % B, l = e / B;
+      double fr = 0.0, fi = 0.0;
+      // F_IJ[i][l0 + l]: R[cI + i, cJ + l0 + l], or conj(R[cJ + l0 + l, cI + i]) for R^H.
+      const int ri = CT ? cJ + l0 + l : cI + i;
+      const int rl = CT ? cI + i : cJ + l0 + l;
+      if (ri < k && rl < k) {
+        const int64_t idx = (int64_t)ri + (int64_t)rl * ldr;
+        fr = (double)R[CPLX ? 2 * idx : idx];
+        fi = CPLX ? (double)R[2 * idx + 1] : 0.0;
+        if (CT) fi = -fi;
+      }
+      f_re[i][l] = fr;
+      f_im[i][l] = fi;
+      // M_JJ[l0 + l][i] (here i runs over the columns of M_JJ).
+      const int64_t at = (int64_t)(cJ + l0 + l) + (int64_t)(cJ + i) * kp;
+      d_re[l][i] = (double)M[at * NP];
+      d_im[l][i] = CPLX ? (double)M[at * NP + 1] : 0.0;
+    }
+    __syncthreads();
+    for (int q = 0; q < ROWS; ++q) {
+      const int i = g + (256 / B) * q;
+      for (int l = 0; l < LC; ++l) {
+        const double ar = f_re[i][l], ai = f_im[i][l], br = d_re[l][j], bi = d_im[l][j];
+        sr[q] += ar * br - ai * bi;
+        si[q] += ar * bi + ai * br;
+      }
+    }
+  }
+  for (int q = 0; q < ROWS; ++q) {
+    const int i = g + (256 / B) * q;
+    const int64_t at = (int64_t)(cI + i) + (int64_t)(cJ + j) * kp;
+    M[at * NP] = (T)(-sr[q]);
+    if (CPLX) M[at * NP + 1] = (T)(-si[q]);
+  }
+}
+
+// The solve. CT = false solves X R = Y (blocks left to right); CT = true
+// solves X R^H = Y, whose factor is lower triangular (blocks right to left).
+// Each wave computes a 32 x 32 piece of the workgroup's TW_ROWS x TW_CB block
+// as 2 x 2 matrix-core tiles.
+template <typename TS, typename T, bool CPLX, bool CT>
+__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void trsm_wide_kernel(
+    TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int TW_KC = TrsmSlice<T>::KC;
   constexpr int TW_APT = TrsmSlice<T>::APT;
@@ -164,51 +248,47 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
   const int nj0 = 32 * (w / WAVE_ROWS);
   const int nbk = (k + TW_CB - 1) / TW_CB;
 
+  // Block of step s and the slices of a block: its own columns first, then
+  // the solved ones (left of it for R, right of it for R^H).
+  auto block_of = [&](int s) { return CT ? nbk - 1 - s : s; };
+  auto own_slices = [&](int J) {
+    const int c0 = J * TW_CB;
+    const int nb = (k - c0) < TW_CB ? (k - c0) : TW_CB;
+    return (nb + TW_KC - 1) / TW_KC;
+  };
+  auto nslices = [&](int J) {
+    const int c0 = J * TW_CB;
+    const int solved = CT ? k - (c0 + TW_CB < k ? c0 + TW_CB : k) : c0;
+    return own_slices(J) + (solved + TW_KC - 1) / TW_KC;
+  };
+  auto slice_col = [&](int J, int idx) {
+    const int c0 = J * TW_CB;
+    const int own = own_slices(J);
+    if (idx < own) return c0 + TW_KC * idx;
+    return CT ? c0 + TW_CB + TW_KC * (idx - own) : TW_KC * (idx - own);
+  };
+
   T ar[TW_APT], ai[TW_APT], bre[TW_BPT], bim[TW_BPT];
-  // A slice: columns [kc, kc + 16) of the workgroup's rows of Y (zero at or
-  // past column climit).
-  auto fetch_a = [&](int kc, int climit) {
+  // Slice at column kc for block J: columns [kc, kc + 16) of the
+  // workgroup's rows of Y and rows [kc, kc + 16) x block J's columns of M,
+  // zero past k.
+  auto fetch = [&](int J, int kc) {
+    const int c0 = J * TW_CB;
 #pragma unroll
     for (int e = 0; e < TW_APT; ++e) {
       const int idx = t + TW_THREADS * e;
       const int r = idx % TW_ROWS;
       const int kk = idx / TW_ROWS;
       const int64_t row = row0 + r;
-      load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, row < m && kc + kk < climit, ar[e], ai[e]);
+      load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, row < m && kc + kk < k, ar[e], ai[e]);
     }
-  };
-  // B slice of the update: rows [kc, kc + 16) x columns [c0, c0 + 128) of R
-  // (of R^H when CT), zero past k.
-  auto fetch_r = [&](int kc, int c0) {
 #pragma unroll
     for (int e = 0; e < TW_BPT; ++e) {
       const int idx = t + TW_THREADS * e;
       const int kk = idx % TW_KC;
       const int c = idx / TW_KC;
-      const bool ok = kc + kk < k && c0 + c < k;
-      if (CT) {
-        // Element (kc + kk, c0 + c) of R^H is conj(R[c0 + c, kc + kk]).
-        load_elem<TS, T, CPLX>(R, (int64_t)(c0 + c) + (int64_t)(kc + kk) * ldr, ok, bre[e], bim[e]);
-        bim[e] = -bim[e];
-      } else {
-        load_elem<TS, T, CPLX>(R, (int64_t)(kc + kk) + (int64_t)(c0 + c) * ldr, ok, bre[e], bim[e]);
-      }
-    }
-  };
-  // B slice of the diagonal multiply: rows [kc, kc + 16) of inv(R_JJ) (of
-  // inv(R_JJ)^H = inv(R_JJ^H) when CT).
-  auto fetch_d = [&](int kc, const T *Dj) {
-#pragma unroll
-    for (int e = 0; e < TW_BPT; ++e) {
-      const int idx = t + TW_THREADS * e;
-      const int kk = idx % TW_KC;
-      const int c = idx / TW_KC;
-      if (CT) {
-        load_elem<T, T, CPLX>(Dj, (int64_t)c + (int64_t)(kc + kk) * TW_CB, true, bre[e], bim[e]);
-        bim[e] = -bim[e];
-      } else {
-        load_elem<T, T, CPLX>(Dj, (int64_t)(kc + kk) + (int64_t)c * TW_CB, true, bre[e], bim[e]);
-      }
+      load_elem<T, T, CPLX>(M, (int64_t)(kc + kk) + (int64_t)(c0 + c) * kp, kc + kk < k && c0 + c < k, bre[e],
+                            bim[e]);
     }
   };
   auto stash = [&]() {
@@ -229,62 +309,54 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
       if (CPLX) Bs[NP - 1][kk * TW_LDB + c] = bim[e];
     }
   };
-  auto mma = [&](acc_t(&p1)[2][2], acc_t(&p2)[2][2], acc_t(&p3)[2][2], auto mode) {
-    constexpr int MODE = decltype(mode)::value;
-    if constexpr (CPLX)
-      mma_steps_3m<T, 2, 2, MODE>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0, TW_KC / 4, 1,
-                                  p1, p2, p3);
-    else
-      mma_steps<T, false, 2, 2, MODE>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4, 1, p1,
-                                      p2);
-  };
-  typedef std::integral_constant<int, PROD_SUB> Sub;
-  typedef std::integral_constant<int, PROD_ADD> Add;
 
-  for (int step = 0; step < nbk; ++step) {
-    const int J = CT ? nbk - 1 - step : step;
+  acc_t p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      p1[mb][nb] = Mfma<T>::zero();
+      p2[mb][nb] = Mfma<T>::zero();
+      p3[mb][nb] = Mfma<T>::zero();
+    }
+
+  int step = 0;
+  int J = block_of(0);
+  int idx = 0;
+  int n = nslices(J);
+  fetch(J, slice_col(J, 0));
+  while (true) {
+    const int kc = slice_col(J, idx);
+    __syncthreads();
+    stash();
+    __syncthreads();
+    // Prefetch the next slice, crossing into the next block.
+    {
+      const bool same = idx + 1 < n;
+      const int Jn = same ? J : block_of(step + 1);
+      if (same || step + 1 < nbk) fetch(Jn, slice_col(Jn, same ? idx + 1 : 0));
+    }
     const int c0 = J * TW_CB;
-    const int cend = (c0 + TW_CB < k) ? c0 + TW_CB : k;
-    // Waves whose columns lie past k only stage slices.
-    const bool active = c0 + nj0 < k;
-    // The block's right-hand side seeds the 3M sums: p1 = Y_re, p2 = 0,
-    // p3 = Y_re + Y_im, counted down by the products, so that combine3m
-    // yields Y_J - sum X_I R_IJ.
-    acc_t p1[2][2], p2[2][2], p3[2][2];
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
-          const int col = c0 + nj0 + 16 * nb + li;
-          T re, im;
-          load_elem<TS, T, CPLX>(Y, row + (int64_t)col * ldy, row < m && col < k, re, im);
-          p1[mb][nb][r] = re;
-          p2[mb][nb][r] = T(0);
-          p3[mb][nb][r] = re + im;
-        }
-    // Update from the blocks already solved: those left of J for R, right of
-    // J for R^H.
-    const int kbeg = CT ? cend : 0;
-    const int kend = CT ? k : c0;
-    if (kbeg < kend) {
-      fetch_a(kbeg, kend);
-      fetch_r(kbeg, c0);
+    // Waves whose columns lie past k only stage slices. Within the diagonal
+    // block, M_JJ is triangular: slice rows past the wave's last column add
+    // nothing (before its first column for the lower-triangular M_JJ of R^H).
+    bool contributes = c0 + nj0 < k;
+    if (kc >= c0 && kc < c0 + TW_CB) {
+      const int kl = kc - c0;
+      contributes = contributes && (CT ? (kl + TW_KC > nj0) : (kl < nj0 + 32));
     }
-    for (int kc = kbeg; kc < kend; kc += TW_KC) {
-      __syncthreads();
-      stash();
-      __syncthreads();
-      if (kc + TW_KC < kend) {
-        fetch_a(kc + TW_KC, kend);
-        fetch_r(kc + TW_KC, c0);
-      }
-      if (active) mma(p1, p2, p3, Sub());
+    if (contributes) {
+      if constexpr (CPLX)
+        mma_steps_3m<T, 2, 2, PROD_ADD>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
+                                        TW_KC / 4, 1, p1, p2, p3);
+      else
+        mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
+                                            1, p1, p2);
     }
-    // The updated right-hand side goes back into the workgroup's rows of Y_J,
-    // where the diagonal multiply reads it as its A operand.
+    if (++idx < n) continue;
+    // Block J is solved. Every wave has staged its last slice of Y_J (the
+    // barriers above), so the stores cannot race a load of the unsolved
+    // columns; the next blocks read X_J only after further barriers.
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -298,9 +370,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
           if (row < m && col < k) store_elem<TS, CPLX>(Y, row + (int64_t)col * ldy, (TS)re[r], (TS)im[r]);
         }
       }
-    // Workgroup-scope release/acquire: the stores above are visible to the
-    // loads below.
-    __syncthreads();
+    if (++step == nbk) break;
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -309,54 +379,31 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2
         p2[mb][nb] = Mfma<T>::zero();
         p3[mb][nb] = Mfma<T>::zero();
       }
-    const T *Dj = Dinv + (int64_t)J * TW_CB * TW_CB * NP;
-    const int nbj = cend - c0;
-    fetch_a(c0, cend);
-    fetch_d(0, Dj);
-    for (int kc = 0; kc < nbj; kc += TW_KC) {
-      __syncthreads();
-      stash();
-      __syncthreads();
-      if (kc + TW_KC < nbj) {
-        fetch_a(c0 + kc + TW_KC, cend);
-        fetch_d(kc + TW_KC, Dj);
-      }
-      // inv(R_JJ) is upper triangular: slice rows past the wave's last column
-      // add nothing (lower triangular for R^H: rows before its first column).
-      const bool contributes = CT ? (kc + TW_KC > nj0) : (kc < nj0 + 32);
-      if (active && contributes) mma(p1, p2, p3, Add());
-    }
-    // Every wave has read its last slice of Y_J before any wave overwrites it.
-    __syncthreads();
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb) {
-        acc_t re = p1[mb][nb], im = p2[mb][nb];
-        if constexpr (CPLX) combine3m<PROD_ADD>(p1[mb][nb], p2[mb][nb], p3[mb][nb], re, im);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
-          const int col = c0 + nj0 + 16 * nb + li;
-          if (row < m && col < k) store_elem<TS, CPLX>(Y, row + (int64_t)col * ldy, (TS)re[r], (TS)im[r]);
-        }
-      }
-    // The solved block is read back by the next block's update.
-    __syncthreads();
+    J = block_of(step);
+    idx = 0;
+    n = nslices(J);
   }
 }
 
 template <typename TS, typename T, bool CPLX, bool CT = false>
-static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_t ldr, T *Dinv,
-                       hipStream_t stream) {
+static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_t ldr, T *M, hipStream_t stream) {
   const int nbk = (k + TW_CB - 1) / TW_CB;
-  hipLaunchKernelGGL((trsm_block_inv_kernel<TS, T, CPLX>), dim3(nbk), dim3(TI_THREADS), 0, stream, R, ldr, k, Dinv);
+  const int64_t kp = trsm_kp(k);
+  hipLaunchKernelGGL((trsm_block_inv_kernel<TS, T, CPLX, CT>), dim3(nbk), dim3(TI_THREADS), 0, stream, R, ldr, k, M,
+                     kp);
   int st = check_launch("trsm_block_inv_kernel");
-  if (st || m == 0) return st;
+  if (st) return st;
+  const int npairs = nbk * (nbk - 1) / 2;
+  if (npairs > 0) {
+    hipLaunchKernelGGL((trsm_offdiag_kernel<TS, T, CPLX, CT>), dim3(npairs), dim3(256), 0, stream, R, ldr, k, M, kp);
+    st = check_launch("trsm_offdiag_kernel");
+    if (st) return st;
+  }
+  if (m == 0) return FHIP_OK;
   const int64_t grid = cdiv64(m, TW_ROWS);
   if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
-  hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y,
-                     ldy, m, k, R, ldr, (const T *)Dinv);
+  hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y, ldy,
+                     m, k, (const T *)M, kp);
   return check_launch("trsm_wide_kernel");
 }
 
@@ -368,7 +415,9 @@ extern "C" int fhip_trsm_workspace_bytes(size_t *nbytes, int64_t k, int compute)
   if (!nbytes) return fail(FHIP_E_ARGUMENT, "fhip_trsm_workspace_bytes: null output pointer");
   if (!valid_type(compute)) return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runn: unknown type %d", compute);
   if (k < 0) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: negative dimension");
-  *nbytes = (size_t)cdiv64(k, TW_CB) * TW_CB * TW_CB * eltype_size(compute);
+  if (k > (1 << 16)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: too many columns (%lld)", (long long)k);
+  const int64_t kp = trsm_kp(k);
+  *nbytes = (size_t)kp * (size_t)kp * eltype_size(compute);
   return FHIP_OK;
 }
 
@@ -381,7 +430,7 @@ extern "C" int fhip_trsm_runn(void *Y, int64_t ldy, int64_t m, int64_t k, const 
                 compute);
   if (m < 0 || k < 0) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: negative dimension");
   if (k == 0) return FHIP_OK;
-  if (k > (1 << 20)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: too many columns");
+  if (k > (1 << 16)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: too many columns");
   if (!Y || !R) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: null matrix pointer");
   if (ldr < k || (m > 0 && ldy < m))
     return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: leading dimension smaller than the matrix");
@@ -412,7 +461,7 @@ extern "C" int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const 
   if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runc: unknown type %d", type);
   if (m < 0 || k < 0) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: negative dimension");
   if (k == 0) return FHIP_OK;
-  if (k > (1 << 20)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: too many columns");
+  if (k > (1 << 16)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: too many columns");
   if (!Y || !R) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: null matrix pointer");
   if (ldr < k || (m > 0 && ldy < m))
     return fail(FHIP_E_ARGUMENT, "fhip_trsm_runc: leading dimension smaller than the matrix");
