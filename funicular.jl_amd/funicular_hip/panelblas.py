@@ -292,13 +292,14 @@ def cholesky_upper_fp32(G, N, k, shift="adaptive"):
     return R, s
 
 
-def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp64"):
+def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp32"):
     """The mixed first pass: FP32 matrix-core Gram with FP64 sums across row
     chunks and an FP32 Cholesky (shifted when needed), then the triangular
-    solve on the FP64 data. solve="fp64" solves in FP64 with the FP32-derived
-    factor, so Q R reproduces Y to FP64 rounding; solve="fp32" runs the solve's
-    products on the FP32 matrix cores, which leaves ||Y - Q R|| / ||Y|| near
-    u32 kappa(Y). Returns (R, shift)."""
+    solve on the FP64 data. solve="fp32" runs the solve's products on the FP32
+    matrix cores, which leaves ||Y - Q R|| / ||Y|| near u32 kappa(Y) (Q stays
+    orthonormal to FP64 level after the FP64 pass); solve="fp64" solves in
+    FP64 with the FP32-derived factor, so Q R reproduces Y to FP64 rounding.
+    Returns (R, shift)."""
     if Y.dtype != np.dtype(np.complex128):
         raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
     if solve not in ("fp64", "fp32"):
@@ -310,7 +311,7 @@ def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp64"):
     return R, s
 
 
-def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp64"):
+def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32"):
     """cholqr2 with a mixed-precision first pass (FP32 Gram and Cholesky, the
     solve in the precision `solve` names) as a preconditioner and the second
     pass in FP64. A third FP64 pass runs when either earlier Cholesky needed a
