@@ -121,6 +121,14 @@ __device__ inline void mma_steps(const T *__restrict__ As_re, const T *__restric
   }
 }
 
+// LDS column swizzle for [k][column] operand planes whose rows are 16 or
+// more columns wide: column c of row kr sits at c ^ lds_swizzle(kr). It flips
+// bit 4 between even and odd rows, so the two rows a half wave reads for one
+// matrix-core operand land in different halves of the banks, and spreads the
+// 16 rows of a slice over the even offsets below 32, so a half wave storing a
+// 16-row x 2-column piece of a slice touches 32 different bank pairs.
+__device__ inline int lds_swizzle(int kr) { return ((kr & 1) << 4) | (((kr >> 1) & 7) << 1); }
+
 // The same K steps for complex operands with three real products per complex
 // product instead of four (the 3M method; Higham, "Stability of a method for
 // multiplying complex matrices with three real matrix multiplications", SIMAX
@@ -131,7 +139,7 @@ __device__ inline void mma_steps(const T *__restrict__ As_re, const T *__restric
 // count down from whatever the accumulators started at. The error bound is
 // normwise, |error| <= c u |a| |b| per product, the same order as the
 // four-product form's for these column-scaled sums.
-template <typename T, int MB, int NB, int MODE>
+template <typename T, int MB, int NB, int MODE, bool SWZ = false>
 __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__restrict__ As_im, int lda_s,
                                     int mi0, const T *__restrict__ Bs_re, const T *__restrict__ Bs_im,
                                     int ldb_s, int nj0, int s_begin, int s_end, int s_stride,
@@ -142,11 +150,12 @@ __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__rest
   const int lk = lane >> 4;
   for (int s = s_begin; s < s_end; s += s_stride) {
     const int kr = 4 * s + lk;
+    const int sw = SWZ ? lds_swizzle(kr) : 0;
     T ar[MB], ai[MB], as[MB], br[NB], bi[NB], bt[NB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {
-      ar[mb] = As_re[kr * lda_s + mi0 + 16 * mb + li];
-      ai[mb] = As_im[kr * lda_s + mi0 + 16 * mb + li];
+      ar[mb] = As_re[kr * lda_s + ((mi0 + 16 * mb + li) ^ sw)];
+      ai[mb] = As_im[kr * lda_s + ((mi0 + 16 * mb + li) ^ sw)];
       as[mb] = ar[mb] + ai[mb];
       if (MODE == PROD_SUB) {
         ar[mb] = -ar[mb];
@@ -156,8 +165,8 @@ __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__rest
     }
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
-      br[nb] = Bs_re[kr * ldb_s + nj0 + 16 * nb + li];
-      bi[nb] = Bs_im[kr * ldb_s + nj0 + 16 * nb + li];
+      br[nb] = Bs_re[kr * ldb_s + ((nj0 + 16 * nb + li) ^ sw)];
+      bi[nb] = Bs_im[kr * ldb_s + ((nj0 + 16 * nb + li) ^ sw)];
       bt[nb] = (MODE == PROD_CONJ_A) ? br[nb] - bi[nb] : br[nb] + bi[nb];
     }
 #pragma unroll

@@ -43,6 +43,9 @@ struct GramShape<128> {
 #ifndef FHIP_GRAM_FP32_TILE
 #define FHIP_GRAM_FP32_TILE 64
 #endif
+#ifndef FHIP_GRAM_SWIZZLE
+#define FHIP_GRAM_SWIZZLE 1
+#endif
 
 template <>
 struct GramShape<64> {
@@ -145,7 +148,11 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
   constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
   constexpr int MB = GramShape<TILE>::MB;
   constexpr int NB = GramShape<TILE>::NB;
-  constexpr int LDS_W = TILE + 4;
+  // Complex FP64 tiles of 32 or more columns use the swizzled LDS layout (no
+  // padding); the others pad each row by 4. The swizzle is laid out for 8-byte
+  // elements: with FP32 planes it measured slower than the padding.
+  constexpr bool SWZ = CPLX && sizeof(TC) == 8 && FHIP_GRAM_SWIZZLE && TILE >= 32;
+  constexpr int LDS_W = SWZ ? TILE : TILE + 4;
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int PER_THREAD = KC * TILE / GRAM_THREADS;
   __shared__ TC As[NP][KC * LDS_W];
@@ -179,6 +186,10 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
   const int nj0 = (KSPLIT == 1) ? (TILE / 2) * (w >> 1) : 0;
   const int s_begin = (KSPLIT > 1) ? w : 0;
   const int s_stride = KSPLIT;
+  // In a diagonal tile of A^H A the quadrant strictly below the diagonal is
+  // never read (the reduce mirrors the upper triangle), so its wave only
+  // stages slices.
+  const bool skip_quadrant = same && KSPLIT == 1 && mi0 > nj0;
 
   typedef typename Mfma<TC>::acc acc_t;
   // Complex tiles keep the three 3M sums (p1, p2, p3); real tiles use p1 only.
@@ -217,20 +228,22 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
       const int idx = t + GRAM_THREADS * e;
       const int kk = idx % KC;
       const int c = idx / KC;
-      As[0][kk * LDS_W + c] = pa_re[e];
-      if (CPLX) As[NP - 1][kk * LDS_W + c] = pa_im[e];
+      const int at = kk * LDS_W + (SWZ ? (c ^ lds_swizzle(kk)) : c);
+      As[0][at] = pa_re[e];
+      if (CPLX) As[NP - 1][at] = pa_im[e];
       if (!same) {
-        Bs[0][kk * LDS_W + c] = pb_re[e];
-        if (CPLX) Bs[NP - 1][kk * LDS_W + c] = pb_im[e];
+        Bs[0][at] = pb_re[e];
+        if (CPLX) Bs[NP - 1][at] = pb_im[e];
       }
     }
     __syncthreads();
     if (rb + KC < r1) fetch(rb + KC);
     const TC *bre = same ? As[0] : Bs[0];
     const TC *bim = same ? As[NP - 1] : Bs[NP - 1];
+    if (skip_quadrant) continue;
     if constexpr (CPLX)
-      mma_steps_3m<TC, MB, NB, PROD_CONJ_A>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
-                                            KC / 4, s_stride, p1, p2, p3);
+      mma_steps_3m<TC, MB, NB, PROD_CONJ_A, SWZ>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
+                                                 KC / 4, s_stride, p1, p2, p3);
     else
       mma_steps<TC, false, MB, NB, PROD_CONJ_A>(As[0], As[0], LDS_W, mi0, bre, bre, LDS_W, nj0, s_begin,
                                                 KC / 4, s_stride, p1, p2);

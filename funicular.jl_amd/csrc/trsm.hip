@@ -42,8 +42,9 @@ namespace fhip {
 constexpr int TW_THREADS = 256;  // 4 waves, each a 32 x 32 tile of the block
 constexpr int TW_CB = FHIP_TRSM_CB;
 constexpr int TW_ROWS = 4096 / TW_CB;
-constexpr int TW_LDA = TW_ROWS + 4;
-constexpr int TW_LDB = TW_CB + 4;
+#ifndef FHIP_TRSM_SWIZZLE
+#define FHIP_TRSM_SWIZZLE 1
+#endif
 constexpr int TI_THREADS = 8 * TW_CB;  // block inverse: 8 row groups x TW_CB columns
 constexpr int TI_RPT = TW_CB / 8;      // rows of the inverse per thread
 constexpr int TI_CHUNK = 16;           // columns of R staged in LDS at a time
@@ -233,6 +234,11 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
   constexpr int TW_KC = TrsmSlice<T>::KC;
   constexpr int TW_APT = TrsmSlice<T>::APT;
   constexpr int TW_BPT = TrsmSlice<T>::BPT;
+  // Complex FP64 planes use the swizzled LDS layout (laid out for 8-byte
+  // elements); the others pad each row by 4.
+  constexpr bool SWZ = CPLX && sizeof(T) == 8 && FHIP_TRSM_SWIZZLE;
+  constexpr int TW_LDA = SWZ ? TW_ROWS : TW_ROWS + 4;
+  constexpr int TW_LDB = SWZ ? TW_CB : TW_CB + 4;
   __shared__ T As[NP][TW_KC * TW_LDA];
   __shared__ T Bs[NP][TW_KC * TW_LDB];
   typedef typename Mfma<T>::acc acc_t;
@@ -297,16 +303,18 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       const int idx = t + TW_THREADS * e;
       const int r = idx % TW_ROWS;
       const int kk = idx / TW_ROWS;
-      As[0][kk * TW_LDA + r] = ar[e];
-      if (CPLX) As[NP - 1][kk * TW_LDA + r] = ai[e];
+      const int at = kk * TW_LDA + (SWZ ? (r ^ lds_swizzle(kk)) : r);
+      As[0][at] = ar[e];
+      if (CPLX) As[NP - 1][at] = ai[e];
     }
 #pragma unroll
     for (int e = 0; e < TW_BPT; ++e) {
       const int idx = t + TW_THREADS * e;
       const int kk = idx % TW_KC;
       const int c = idx / TW_KC;
-      Bs[0][kk * TW_LDB + c] = bre[e];
-      if (CPLX) Bs[NP - 1][kk * TW_LDB + c] = bim[e];
+      const int at = kk * TW_LDB + (SWZ ? (c ^ lds_swizzle(kk)) : c);
+      Bs[0][at] = bre[e];
+      if (CPLX) Bs[NP - 1][at] = bim[e];
     }
   };
 
@@ -347,8 +355,8 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     }
     if (contributes) {
       if constexpr (CPLX)
-        mma_steps_3m<T, 2, 2, PROD_ADD>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
-                                        TW_KC / 4, 1, p1, p2, p3);
+        mma_steps_3m<T, 2, 2, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
+                                             TW_KC / 4, 1, p1, p2, p3);
       else
         mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
                                             1, p1, p2);
