@@ -88,12 +88,40 @@ def cholqr2(Y, *, nbuffers=None):
     A pass whose Gram needed a shift triggers a third pass (shifted
     CholeskyQR3, Fukaya et al.)."""
     R, shifted = cholqr_pass(Y, nbuffers)
-    R2, shifted2 = cholqr_pass(Y, nbuffers)
-    R = R2 @ R
+    R, shifted2 = cholqr_pass_then(Y, R, nbuffers)
     if shifted or shifted2:
-        R3, _ = cholqr_pass(Y, nbuffers)
-        R = R3 @ R
+        R, _ = cholqr_pass_then(Y, R, nbuffers)
     return R
+
+
+def triangular_product(A, B):
+    """A B for upper triangular A and B (BLAS trmm: half the flops of a full
+    product; the result is upper triangular)."""
+    A = np.asfortranarray(A)
+    B = np.asfortranarray(B, dtype=A.dtype)
+    trmm, = scipy.linalg.get_blas_funcs(("trmm",), (A, B))
+    return trmm(1.0, A, B, side=0, lower=0, trans_a=0, diag=0)
+
+
+_product_pool = None
+
+
+def cholqr_pass_then(Y, R_prev, nbuffers=None):
+    """One more CholQR pass over Y, returning (R_pass R_prev, shifted). The
+    host forms the triangular product while the pass's solve sweep runs on the
+    device, instead of after it."""
+    global _product_pool
+    G = gram(Y, nbuffers=nbuffers)
+    R, shifted = cholesky_upper(G, Y.N, Y.k)
+    if _product_pool is None:
+        import concurrent.futures
+        _product_pool = concurrent.futures.ThreadPoolExecutor(max_workers=1, thread_name_prefix="cholqr-product")
+    product = _product_pool.submit(triangular_product, R, R_prev)
+    try:
+        rdiv_rows(Y, R, nbuffers=nbuffers)
+    finally:
+        total = product.result()
+    return total, shifted
 
 
 def cholqr_pass(Y, nbuffers=None):
@@ -108,10 +136,11 @@ def _potrf(G):
     """Upper Cholesky factor of the Hermitian matrix whose upper triangle G
     holds (LAPACK potrf, as cholesky(Hermitian(G)) does); None if it fails."""
     potrf, = scipy.linalg.get_lapack_funcs(("potrf",), (G,))
+    # clean=True zeroes the strictly lower triangle of the factor.
     c, info = potrf(G, lower=False, clean=True, overwrite_a=False)
     if info != 0:
         return None
-    return np.asfortranarray(np.triu(c))
+    return np.asfortranarray(c)
 
 
 def cholesky_upper(G, N, k):
@@ -319,9 +348,7 @@ def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32"):
     with a condition number well above 1, which one FP64 pass does not bring
     to FP64 orthogonality."""
     R1, s = cholqr_pass_mixed(Y, nbuffers, shift, solve)
-    R2, shifted = cholqr_pass(Y, nbuffers)
-    R = R2 @ R1
+    R, shifted = cholqr_pass_then(Y, R1, nbuffers)
     if shifted or s > 0.0:
-        R3, _ = cholqr_pass(Y, nbuffers)
-        R = R3 @ R
+        R, _ = cholqr_pass_then(Y, R, nbuffers)
     return R
