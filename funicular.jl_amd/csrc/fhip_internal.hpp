@@ -226,6 +226,29 @@ __device__ inline void load_elem(const TIn *__restrict__ p, int64_t idx, bool ok
   }
 }
 
+// Branch-free load of one element as it is stored, for staging registers: a
+// lane outside the matrix reads element 0 instead (always a valid address),
+// and from_raw() zeroes it. Keeping the conversion to the compute type out of
+// the load lets the load stay in flight until the value is staged; a
+// conversion next to a guarded load makes the compiler wait for each load on
+// the spot.
+template <typename TIn, bool CPLX>
+__device__ inline typename Pair<TIn>::type load_raw(const TIn *__restrict__ p, int64_t idx, bool ok) {
+  typedef typename Pair<TIn>::type P2;
+  const int64_t i = ok ? idx : 0;
+  if (CPLX) return reinterpret_cast<const P2 *>(p)[i];
+  P2 v;
+  v.x = p[i];
+  v.y = TIn(0);
+  return v;
+}
+
+template <typename T, typename P2>
+__device__ inline void from_raw(const P2 &v, bool ok, T &re, T &im) {
+  re = ok ? T(v.x) : T(0);
+  im = ok ? T(v.y) : T(0);
+}
+
 // Stores one element of a real or complex matrix.
 template <typename TOut, bool CPLX>
 __device__ inline void store_elem(TOut *__restrict__ p, int64_t idx, TOut re, TOut im) {

@@ -18,6 +18,7 @@
 #include "fhip_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace fhip {
 
@@ -204,7 +205,18 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
     }
 
   // Register staging of the next slice while the current one is multiplied.
+  // Unconverted data is loaded straight into the compute registers, and a
+  // diagonal tile loads its one operand once. When the stored values are
+  // converted (FP32 products of FP64 data) the registers hold the raw stored
+  // pairs, converted only when staged, and both operands are loaded on every
+  // tile (the second load of a diagonal tile's rows hits the cache): a
+  // conversion next to a guarded load, or a branch on the tile kind around
+  // the loads, makes the compiler wait for each load on the spot.
+  constexpr bool CONVERTED = !std::is_same<TIn, TC>::value;
+  typedef typename Pair<TIn>::type P2;
   TC pa_re[PER_THREAD], pa_im[PER_THREAD], pb_re[PER_THREAD], pb_im[PER_THREAD];
+  P2 ra[PER_THREAD], rbv[PER_THREAD];
+  bool oka[PER_THREAD], okb[PER_THREAD];
   auto fetch = [&](int64_t rb) {
 #pragma unroll
     for (int e = 0; e < PER_THREAD; ++e) {
@@ -213,12 +225,20 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
       const int c = idx / KC;
       const int64_t row = rb + kk;
       const bool rok = row < r1;
-      load_elem<TIn, TC, CPLX>(A, row + (int64_t)(i0 + c) * lda, rok && (i0 + c) < ka, pa_re[e], pa_im[e]);
-      if (!same)
-        load_elem<TIn, TC, CPLX>(B, row + (int64_t)(j0 + c) * ldb, rok && (j0 + c) < kb, pb_re[e],
-                                 pb_im[e]);
+      if constexpr (CONVERTED) {
+        oka[e] = rok && (i0 + c) < ka;
+        ra[e] = load_raw<TIn, CPLX>(A, row + (int64_t)(i0 + c) * lda, oka[e]);
+        okb[e] = rok && (j0 + c) < kb;
+        rbv[e] = load_raw<TIn, CPLX>(B, row + (int64_t)(j0 + c) * ldb, okb[e]);
+      } else {
+        load_elem<TIn, TC, CPLX>(A, row + (int64_t)(i0 + c) * lda, rok && (i0 + c) < ka, pa_re[e], pa_im[e]);
+        if (!same)
+          load_elem<TIn, TC, CPLX>(B, row + (int64_t)(j0 + c) * ldb, rok && (j0 + c) < kb, pb_re[e],
+                                   pb_im[e]);
+      }
     }
   };
+  const bool b_staged = CONVERTED || !same;
 
   if (r0 < r1) fetch(r0);
   for (int64_t rb = r0; rb < r1; rb += KC) {
@@ -229,17 +249,27 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
       const int kk = idx % KC;
       const int c = idx / KC;
       const int at = kk * LDS_W + (SWZ ? (c ^ lds_swizzle(kk)) : c);
-      As[0][at] = pa_re[e];
-      if (CPLX) As[NP - 1][at] = pa_im[e];
-      if (!same) {
-        Bs[0][at] = pb_re[e];
-        if (CPLX) Bs[NP - 1][at] = pb_im[e];
+      if constexpr (CONVERTED) {
+        TC re, im;
+        from_raw(ra[e], oka[e], re, im);
+        As[0][at] = re;
+        if (CPLX) As[NP - 1][at] = im;
+        from_raw(rbv[e], okb[e], re, im);
+        Bs[0][at] = re;
+        if (CPLX) Bs[NP - 1][at] = im;
+      } else {
+        As[0][at] = pa_re[e];
+        if (CPLX) As[NP - 1][at] = pa_im[e];
+        if (!same) {
+          Bs[0][at] = pb_re[e];
+          if (CPLX) Bs[NP - 1][at] = pb_im[e];
+        }
       }
     }
     __syncthreads();
     if (rb + KC < r1) fetch(rb + KC);
-    const TC *bre = same ? As[0] : Bs[0];
-    const TC *bim = same ? As[NP - 1] : Bs[NP - 1];
+    const TC *bre = b_staged ? Bs[0] : As[0];
+    const TC *bim = b_staged ? Bs[NP - 1] : As[NP - 1];
     if (skip_quadrant) continue;
     if constexpr (CPLX)
       mma_steps_3m<TC, MB, NB, PROD_CONJ_A, SWZ>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,

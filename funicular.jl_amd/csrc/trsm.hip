@@ -274,7 +274,14 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     return CT ? c0 + TW_CB + TW_KC * (idx - own) : TW_KC * (idx - own);
   };
 
+  // Staging registers. Unconverted data is loaded straight into the compute
+  // registers; converted data (FP32 products of FP64 storage) is held as the
+  // raw stored pairs and converted only when staged: a conversion next to a
+  // guarded load makes the compiler wait for each load on the spot.
+  constexpr bool CONVERTED = !std::is_same<TS, T>::value;
   T ar[TW_APT], ai[TW_APT], bre[TW_BPT], bim[TW_BPT];
+  typename Pair<TS>::type ya[TW_APT];
+  bool oka[TW_APT];
   // Slice at column kc for block J: columns [kc, kc + 16) of the
   // workgroup's rows of Y and rows [kc, kc + 16) x block J's columns of M,
   // zero past k.
@@ -286,7 +293,13 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       const int r = idx % TW_ROWS;
       const int kk = idx / TW_ROWS;
       const int64_t row = row0 + r;
-      load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, row < m && kc + kk < k, ar[e], ai[e]);
+      const bool ok = row < m && kc + kk < k;
+      if constexpr (CONVERTED) {
+        oka[e] = ok;
+        ya[e] = load_raw<TS, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, ok);
+      } else {
+        load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, ok, ar[e], ai[e]);
+      }
     }
 #pragma unroll
     for (int e = 0; e < TW_BPT; ++e) {
@@ -304,6 +317,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       const int r = idx % TW_ROWS;
       const int kk = idx / TW_ROWS;
       const int at = kk * TW_LDA + (SWZ ? (r ^ lds_swizzle(kk)) : r);
+      if constexpr (CONVERTED) from_raw(ya[e], oka[e], ar[e], ai[e]);
       As[0][at] = ar[e];
       if (CPLX) As[NP - 1][at] = ai[e];
     }
