@@ -21,7 +21,7 @@ import numpy as np
 from . import plan as _plan
 from .executor import readpanels, rowtraversal, sweep, updatepanels, writepanels
 from .panelblas import cholqr2, gram, panelmul, project, rightmul
-from .panels import PanelMatrix, assert_same_rows, similar
+from .panels import PanelMatrix, assert_same_rows, pooled_like, similar
 
 
 def copyltu_real_diag(M):
@@ -82,21 +82,22 @@ def rrule_project(Q, G, *, nbuffers=None):
         Sbar = np.asarray(Sbar, dtype=Q.dtype)
         plan = Q.plan
         backend = plan.backend
-        Z1 = similar(Q)
-        Z2 = similar(Q)
-        rightmul(Z1, Q, Sbar.conj().T, nbuffers=nbuffers)
-        rightmul(Z2, Q, Sbar, nbuffers=nbuffers)
-        dbar = backend.alloc_device(np.float64, (Q.N, 1))
-
-        def rows(j, Qb, Zb):
-            backend.rowdot(dbar, Qb, Zb, accumulate=j > 0)
-        sweep(rows, readpanels(Q), readpanels(Z1), nbuffers=nbuffers)
         Qbar = similar(Q)
-        panelmul(Qbar, G, Z1, nbuffers=nbuffers)
-        W = Z1  # reused: Z1 is no longer needed
-        panelmul(W, G.adjoint(), Z2, nbuffers=nbuffers)
-        sweep(lambda _, Wb, Qb: backend.axpby(Qb, Wb, 1.0, 1.0), readpanels(W), updatepanels(Qbar),
-              nbuffers=nbuffers)
+        dbar = backend.alloc_device(np.float64, (Q.N, 1))
+        # The two products with Sbar are scratch: under a device-resident plan
+        # they come out of (and go back to) the plan's device pool.
+        with pooled_like(Q) as Z1, pooled_like(Q) as Z2:
+            rightmul(Z1, Q, Sbar.conj().T, nbuffers=nbuffers)
+            rightmul(Z2, Q, Sbar, nbuffers=nbuffers)
+
+            def rows(j, Qb, Zb):
+                backend.rowdot(dbar, Qb, Zb, accumulate=j > 0)
+            sweep(rows, readpanels(Q), readpanels(Z1), nbuffers=nbuffers)
+            panelmul(Qbar, G, Z1, nbuffers=nbuffers)
+            W = Z1  # reused: Z1 is no longer needed
+            panelmul(W, G.adjoint(), Z2, nbuffers=nbuffers)
+            sweep(lambda _, Wb, Qb: backend.axpby(Qb, Wb, 1.0, 1.0), readpanels(W), updatepanels(Qbar),
+                  nbuffers=nbuffers)
         return Qbar, backend.to_host(dbar).ravel()
 
     return S, pullback

@@ -577,6 +577,46 @@ def copyto(dest, src):
     return copycols(dest, range(dest.k), src)
 
 
+class pooled_like:
+    """A context manager giving a scratch PanelMatrix shaped like pm. Under a
+    device-resident plan its storage is an N x k buffer checked out of the
+    plan's device pool and checked back in on exit, so repeated calls reuse
+    one allocation instead of allocating and freeing gigabytes each time;
+    other plans get an ordinary PanelMatrix, freed on exit."""
+
+    def __init__(self, pm):
+        self.pm = pm
+        self.buffer = None
+        self.scratch = None
+
+    def __enter__(self):
+        pm = self.pm
+        plan = pm.plan
+        if not plan.device_resident:
+            self.scratch = similar(pm)
+            return self.scratch
+        self.buffer = _plan.checkout_device_buffers(plan, pm.dtype, (pm.N, pm.k), 1)[0]
+        panels = []
+        for j in range(npanels(pm)):
+            r = panelrange(pm, j)
+            panels.append(Panel(DeviceStorage(self.buffer.view(None, (r.start, r.stop))), DeviceTier(plan.backend),
+                                None, len(r)))
+        self.scratch = PanelMatrix(pm.N, pm.k, plan=plan, w=pm.w, dtype=pm.dtype, _panels=panels,
+                                   _device=self.buffer)
+        return self.scratch
+
+    def __exit__(self, *exc):
+        if self.buffer is not None:
+            for p in self.scratch.panels:
+                p.data = None
+            self.scratch.device = None
+            _plan.checkin_device_buffers(self.pm.plan, [self.buffer])
+        elif self.scratch is not None:
+            free(self.scratch)
+        self.scratch = None
+        return False
+
+
 def similar(pm, dtype=None, dims=None):
     """src/panels.jl:322-344: the width carries over, clamped to the new k."""
     T = pm.dtype if dtype is None else np.dtype(dtype)

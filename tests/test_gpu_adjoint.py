@@ -137,8 +137,8 @@ def test_cholqr2_pullback_real(backend):
     assert OB.isapprox(Ybar, np.real(OA.qr_pullback(Q, R, Qb, Rb)), 1e-11)
 
 
-@pytest.mark.parametrize("w", (16, 5))
-def test_project_pullback_against_oracle(backend, w):
+@pytest.mark.parametrize("w,resident", [(16, False), (5, False), (16, True)])
+def test_project_pullback_against_oracle(backend, w, resident):
     rng = np.random.default_rng(50 + w)
     N, k = 4096, 16
     d0, o0 = 2 + 0.1j, -1.0
@@ -146,10 +146,14 @@ def test_project_pullback_against_oracle(backend, w):
     G = F.Tridiagonal1D(N, d0, o0, diag, backend=backend)
     Qh = randmatrix(rng, N, k)
     Sb = randmatrix(rng, k, k)
-    plan = make_plan(backend)
+    plan = make_plan(backend, device_resident=True) if resident else make_plan(backend)
     Q = F.PanelMatrix.from_array(Qh, plan=plan, w=w)
     S, pullback = F.rrule_project(Q, G)
     Qbar, dbar = pullback(Sb)
+    if resident:
+        # A second call reuses the pooled scratch and gives the same bits.
+        Qbar2, dbar2 = pullback(Sb)
+        assert np.array_equal(F.to_matrix(Qbar2), F.to_matrix(Qbar)) and np.array_equal(dbar2, dbar)
     Qr, dr = OA.project_pullback(Qh, lambda X: OO.tridiag_apply(X, d0, o0, diag),
                                  lambda X: OO.tridiag_apply(X, d0, o0, diag, adjoint=True), Sb)
     assert OB.isapprox(F.to_matrix(Qbar), Qr, 1e-12)
