@@ -402,7 +402,9 @@ def kernels_only(rows, k, reps):
     """The per-launch shapes of the benchmark, without the pipeline around
     them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
     for the HBM counters), then `reps` FP64 Grams, FP32 (mixed) Grams, FP64 and
-    FP32 triangular solves on it. For profiling runs."""
+    FP32 triangular solves, and FP64 row-block products by a k x k factor on
+    it. Prints each kernel's average duration from HIP events (one JSON line).
+    For profiling runs."""
     import funicular_hip as F
     backend = F.hip_backend()
     A = backend.alloc_device(np.complex128, (rows, k))
@@ -411,16 +413,41 @@ def kernels_only(rows, k, reps):
     backend.ghost_fill(A, 0, 1, 7)
     backend.convert_copy(B, A)
     backend.sync_all()
-    for compute in (None, np.complex64):
+    import ctypes
+    from funicular_hip import _lib
+    out = {"rows": rows, "k": k}
+
+    def event():
+        e = ctypes.c_void_p()
+        _lib.call("fhip_event_create", ctypes.byref(e), 0)
+        return e
+
+    def timed(name, fn, flops):
+        # The launches go to the default stream, where the events are recorded.
+        fn()  # untimed first launch
+        backend.sync_all()
+        e0, e1 = event(), event()
+        _lib.call("fhip_event_record", e0, None)
         for _ in range(reps):
-            backend.gram_accumulate(C, A, A, 1.0, 0.0, compute=compute)
+            fn()
+        _lib.call("fhip_event_record", e1, None)
+        _lib.call("fhip_event_sync", e1)
+        ms = ctypes.c_float()
+        _lib.call("fhip_event_elapsed", ctypes.byref(ms), e0, e1)
+        for e in (e0, e1):
+            _lib.call("fhip_event_destroy", e)
+        out[name] = {"ms": ms.value / reps, "tflops": flops / (ms.value / reps) / 1e9}
+
+    for compute, name in ((None, "gram_fp64"), (np.complex64, "gram_fp32")):
+        timed(name, lambda: backend.gram_accumulate(C, A, A, 1.0, 0.0, compute=compute), gram_flops(rows, k))
     G = backend.to_host(C)
     R = backend.todevice(np.asfortranarray(np.linalg.cholesky(G).conj().T))
-    for compute in (None, np.complex64):
-        for _ in range(reps):
-            backend.rdiv_upper(B, R, compute=compute)
+    for compute, name in ((None, "trsm_fp64"), (np.complex64, "trsm_fp32")):
+        timed(name, lambda: backend.rdiv_upper(B, R, compute=compute), trsm_flops(rows, k))
+    timed("rightmul_fp64", lambda: backend.rightmul_gemm(B, A, R), 8.0 * rows * k * k)
     backend.sync_all()
     log(f"kernels-only: {rows}x{k}, {reps} repetitions of each kernel")
+    print(json.dumps(out), flush=True)
 
 
 def streamed(args):
