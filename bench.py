@@ -97,7 +97,7 @@ class KernelTimer:
 
 
 def measured_traffic(kind):
-    """HBM bytes per launch of the configuration-2 row-block kernels, as the
+    """HBM bytes per launch of the 629,856 x 512 row-block kernels, as the
     FETCH_SIZE / WRITE_SIZE counter runs measured them (scaled by a plain copy
     kernel); the summary, with its commands, is in
     profiles/hbm_traffic_rowblock_629856x512.md."""
@@ -462,7 +462,7 @@ def streamed(args):
     from funicular_hip.executor import readpanels, rowtraversal, sweep
     backend = F.hip_backend()
     # The GPU boxes this was measured on cap page-locked host memory at 64 GiB,
-    # below configuration 2's 82.6 GB basis; the streamed run takes the largest
+    # below the 216^3 x 512 basis (82.6 GB); the streamed run takes the largest
     # grid whose basis fits (180^3 rows: 47.8 GB) under a 60 GiB host budget.
     n = args.stream_grid
     N, k, w = n ** 3, args.k, args.w
@@ -640,7 +640,7 @@ def main():
                            "R_rel_diff_vs_fp64": float(np.linalg.norm(Rm - R64) / np.linalg.norm(R64)),
                            "fp64_solve": {"rows_per_s": N * args.steps / dts, "speedup_vs_fp64": dt64 / dts,
                                           "orthogonality": orths, "residual": ress}}
-    # The configuration-2 basis is done with: release it before the other
+    # The 216^3 x 512 basis is done with: release it before the other
     # workloads allocate theirs.
     del S
     F.free(Y)

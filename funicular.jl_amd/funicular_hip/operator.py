@@ -119,8 +119,8 @@ class DenseOperator:
 
 class Tridiagonal1D:
     """Periodic tridiagonal stencil (Gx)_i = (d + diag_i) x_i + o (x_{i-1} + x_{i+1}),
-    indices mod N. Configuration 1 uses d = 2+0.1i, o = -1; configuration 4
-    adds the real parameters diag_i = 1 + 0.5 sin(2 pi i / N)."""
+    indices mod N. The 1-D sketch chain uses d = 2+0.1i, o = -1; the adjoint
+    workload adds the real parameters diag_i = 1 + 0.5 sin(2 pi i / N)."""
 
     panel_capable = True
     workspace_bytes = 0
