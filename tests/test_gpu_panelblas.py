@@ -192,8 +192,8 @@ def test_ghost_feeds_sweeps_and_refuses_writes(backend):
     assert all(p.pins == 0 for p in O.panels)
 
 
+def test_sketch_chain_matches_the_oracle(backend):
     # The 1-D sketch chain at a small N: ghost -> tridiagonal -> cholqr2 -> project.
-    # The 1-D sketch chain (ghost panels through a tridiagonal operator) at a small N: ghost -> tridiagonal -> cholqr2 -> project.
     N, k, w = 4096, 64, 8
     plan = make_plan(backend)
     O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
