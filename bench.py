@@ -402,8 +402,9 @@ def kernels_only(rows, k, reps):
     """The per-launch shapes of the benchmark, without the pipeline around
     them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
     for the HBM counters), then `reps` FP64 Grams, FP32 (mixed) Grams, FP64 and
-    FP32 triangular solves, and FP64 row-block products by a k x k factor on
-    it. Prints each kernel's average duration from HIP events (one JSON line).
+    FP32 triangular solves, FP64 row-block products by a k x k factor, and
+    the elementwise dot / norm / scale / axpby steps on it. Prints each
+    kernel's average duration from HIP events (one JSON line).
     For profiling runs."""
     import funicular_hip as F
     backend = F.hip_backend()
@@ -422,7 +423,7 @@ def kernels_only(rows, k, reps):
         _lib.call("fhip_event_create", ctypes.byref(e), 0)
         return e
 
-    def timed(name, fn, flops):
+    def timed(name, fn, flops, nbytes=None):
         # The launches go to the default stream, where the events are recorded.
         fn()  # untimed first launch
         backend.sync_all()
@@ -436,7 +437,8 @@ def kernels_only(rows, k, reps):
         _lib.call("fhip_event_elapsed", ctypes.byref(ms), e0, e1)
         for e in (e0, e1):
             _lib.call("fhip_event_destroy", e)
-        out[name] = {"ms": ms.value / reps, "tflops": flops / (ms.value / reps) / 1e9}
+        t = ms.value / reps
+        out[name] = {"ms": t, "tflops": flops / t / 1e9} if nbytes is None else {"ms": t, "gbps": nbytes / t / 1e6}
 
     for compute, name in ((None, "gram_fp64"), (np.complex64, "gram_fp32")):
         timed(name, lambda: backend.gram_accumulate(C, A, A, 1.0, 0.0, compute=compute), gram_flops(rows, k))
@@ -445,6 +447,15 @@ def kernels_only(rows, k, reps):
     for compute, name in ((None, "trsm_fp64"), (np.complex64, "trsm_fp32")):
         timed(name, lambda: backend.rdiv_upper(B, R, compute=compute), trsm_flops(rows, k))
     timed("rightmul_fp64", lambda: backend.rightmul_gemm(B, A, R), 8.0 * rows * k * k)
+    # The elementwise panel BLAS steps, HBM-bound: GB/s of algorithmic bytes
+    # (each operand read once, the output written once).
+    blk = rows * k * 16
+    work = backend.alloc_device(np.uint8, (backend.dot_workspace_bytes(rows, k), 1))
+    acc = backend.alloc_device(np.complex128, (1, 1))
+    timed("dot_fp64", lambda: backend.dot_accumulate(acc, A, B, False, work), 0, 2 * blk)
+    timed("norm_fp64", lambda: backend.dot_accumulate(acc, A, None, False, work), 0, blk)
+    timed("scale_fp64", lambda: backend.scale(B, 0.6 + 0.8j), 0, 2 * blk)
+    timed("axpby_fp64", lambda: backend.axpby(B, A, 0.5, 1.0), 0, 3 * blk)
     backend.sync_all()
     log(f"kernels-only: {rows}x{k}, {reps} repetitions of each kernel")
     print(json.dumps(out), flush=True)
@@ -630,13 +641,21 @@ def main():
         orthf = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
         resf = residual(backend, Y, Rf, G, omega)
         log(f"bench: mixed with the Fukaya shift: ||Q^H Q - I||_F = {orthf:.2e}, ||Y - QR||/||Y|| = {resf:.2e}")
+        g32 = summarize(km[("gram", True)], gram_flops)
+        t32 = summarize(km[("trsm", True)], trsm_flops)
+        # Per-pass ratio: one row block's Gram + solve with FP64 products against
+        # the same pair with FP32 products, from the live launch timings.
+        pass64 = gram64["avg_ms"] + trsm64["avg_ms"]
+        pass32 = g32["avg_ms"] + t32["avg_ms"]
         record["mixed"] = {"rows_per_s": N * args.steps / dtm, "ms_per_step": dtm / args.steps * 1e3,
                            "speedup_vs_fp64": dt64 / dtm, "orthogonality": orthm, "residual": resm,
                            "first_pass": "FP32 Gram (FP64 chunk sums), FP32 Cholesky, FP32 solve",
                            "shift": "adaptive (unshifted FP32 Cholesky unless it fails)",
                            "fukaya_shift_orthogonality": orthf, "fukaya_shift_residual": resf,
-                           "gram_fp32_tflops": summarize(km[("gram", True)], gram_flops)["tflops"],
-                           "trsm_fp32_tflops": summarize(km[("trsm", True)], trsm_flops)["tflops"],
+                           "gram_fp32_tflops": g32["tflops"], "trsm_fp32_tflops": t32["tflops"],
+                           "first_pass_vs_fp64_pass": {"fp64_gram_plus_solve_ms": pass64,
+                                                       "fp32_gram_plus_solve_ms": pass32,
+                                                       "speedup": pass64 / pass32},
                            "R_rel_diff_vs_fp64": float(np.linalg.norm(Rm - R64) / np.linalg.norm(R64)),
                            "fp64_solve": {"rows_per_s": N * args.steps / dts, "speedup_vs_fp64": dt64 / dts,
                                           "orthogonality": orths, "residual": ress}}

@@ -1,8 +1,10 @@
 // Elementwise device kernels: eltype conversion (convert_copy!,
 // src/kernels.jl:7-20, used by convert_staged! src/panels.jl:365-371 and
-// rightmul! src/panelblas.jl:216), a fill that stands in for memset, and the
+// rightmul! src/panelblas.jl:216), a fill that stands in for memset, the
 // two matrix-free operators of the benchmark configurations, applied to a
-// whole panel (the panel_capable path of apply_operator!, src/operator.jl:80).
+// whole panel (the panel_capable path of apply_operator!, src/operator.jl:80),
+// and the steps of the elementwise panel BLAS (scale!, axpy!, dot, norm,
+// src/panelblas.jl:303-370).
 #include "fhip_internal.hpp"
 
 namespace fhip {
@@ -175,6 +177,110 @@ __global__ __launch_bounds__(256) void rowdot_kernel(double *__restrict__ out, c
   out[i] = s;
 }
 
+// Y <- Y alpha, entry by entry, as Julia's rmul! computes it
+// (src/panelblas.jl:311-316): a real scalar scales both parts, a complex one
+// is the schoolbook product. Contraction is off so each part is one rounded
+// product (and one rounded sum for a complex scalar), as on the CPU.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void scale_kernel(T *__restrict__ Y, int64_t ldy, int64_t rows, T ar, T ai,
+                                                    int real_scalar) {
+#pragma clang fp contract(off)
+  const int64_t j = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < rows; i += (int64_t)gridDim.x * 256) {
+    T yr, yi;
+    load_elem<T, T, C>(Y, i + j * ldy, true, yr, yi);
+    T outr, outi;
+    if (!C || real_scalar) {
+      outr = yr * ar;
+      outi = yi * ar;
+    } else {
+      outr = yr * ar - yi * ai;
+      outi = yr * ai + yi * ar;
+    }
+    store_elem<T, C>(Y, i + j * ldy, outr, outi);
+  }
+}
+
+// Sum over a rows x cols block of conj(x) y (mode 0) or |x|^2 (mode 1), in
+// FP64. Workgroup (chunk, column) sums DOT_ROWS rows of one column, each
+// thread DOT_PER_THREAD of them at a stride of 256, then an LDS tree in a
+// fixed pairing; the partials go to slot column * nchunks + chunk. The order
+// of every addition depends only on (rows, cols).
+constexpr int DOT_THREADS = 256;
+constexpr int DOT_PER_THREAD = 16;
+constexpr int64_t DOT_ROWS = (int64_t)DOT_THREADS * DOT_PER_THREAD;
+
+template <int N>
+__device__ inline void tree_sum(double (&sh)[N][DOT_THREADS], double (&v)[N]) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < N; ++q) sh[q][t] = v[q];
+  __syncthreads();
+  for (int half = DOT_THREADS / 2; half > 0; half >>= 1) {
+    if (t < half) {
+#pragma unroll
+      for (int q = 0; q < N; ++q) sh[q][t] += sh[q][t + half];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool C>
+__global__ __launch_bounds__(DOT_THREADS) void dot_partial_kernel(double *__restrict__ partial,
+                                                                  const T *__restrict__ X, int64_t ldx,
+                                                                  const T *__restrict__ Y, int64_t ldy,
+                                                                  int64_t rows, int64_t nchunks, int mode) {
+  const int64_t chunk = blockIdx.x;
+  const int64_t j = blockIdx.y;
+  const int64_t r0 = chunk * DOT_ROWS;
+  double sre = 0.0, sim = 0.0;
+#pragma unroll 4
+  for (int e = 0; e < DOT_PER_THREAD; ++e) {
+    const int64_t i = r0 + (int64_t)DOT_THREADS * e + threadIdx.x;
+    if (i < rows) {
+      T xr, xi;
+      load_elem<T, T, C>(X, i + j * ldx, true, xr, xi);
+      if (mode == 1) {
+        sre += (double)xr * (double)xr + (double)xi * (double)xi;
+      } else {
+        T yr, yi;
+        load_elem<T, T, C>(Y, i + j * ldy, true, yr, yi);
+        sre += (double)xr * (double)yr + (double)xi * (double)yi;
+        sim += (double)xr * (double)yi - (double)xi * (double)yr;
+      }
+    }
+  }
+  __shared__ double sh[2][DOT_THREADS];
+  double v[2] = {sre, sim};
+  tree_sum<2>(sh, v);
+  if (threadIdx.x == 0) {
+    const int64_t slot = j * nchunks + chunk;
+    partial[2 * slot] = sh[0][0];
+    partial[2 * slot + 1] = sh[1][0];
+  }
+}
+
+// out <- (accumulate ? out : 0) + the sum of n partials, one workgroup:
+// thread t adds slots t, t + 256, ... in order, then the LDS tree.
+__global__ __launch_bounds__(DOT_THREADS) void dot_finish_kernel(double *__restrict__ out,
+                                                                 const double *__restrict__ partial, int64_t n,
+                                                                 int accumulate) {
+  double sre = 0.0, sim = 0.0;
+  for (int64_t q = threadIdx.x; q < n; q += DOT_THREADS) {
+    sre += partial[2 * q];
+    sim += partial[2 * q + 1];
+  }
+  __shared__ double sh[2][DOT_THREADS];
+  double v[2] = {sre, sim};
+  tree_sum<2>(sh, v);
+  if (threadIdx.x == 0) {
+    out[0] = (accumulate ? out[0] : 0.0) + sh[0][0];
+    out[1] = (accumulate ? out[1] : 0.0) + sh[1][0];
+  }
+}
+
+static int64_t dot_chunks(int64_t rows) { return cdiv64(rows, DOT_ROWS); }
+
 }  // namespace fhip
 
 using namespace fhip;
@@ -339,4 +445,90 @@ extern "C" int fhip_stencil_helmholtz3d(void *Y, int64_t ldy, const void *X, int
     hipLaunchKernelGGL((helmholtz_kernel<float>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X,
                        ldx, n1, n2, n3, w, (float)d_re, (float)d_im, (float)o_re, (float)o_im);
   return check_launch("helmholtz_kernel");
+}
+
+extern "C" int fhip_scale(void *Y, int64_t ldy, int64_t rows, int64_t cols, double a_re, double a_im,
+                          int real_scalar, int type, void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_scale: unknown eltype");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_scale: negative dimension");
+  if (rows == 0 || cols == 0) return FHIP_OK;
+  if (!Y) return fail(FHIP_E_ARGUMENT, "fhip_scale: null pointer");
+  if (ldy < rows) return fail(FHIP_E_ARGUMENT, "fhip_scale: leading dimension smaller than the rows");
+  if (cols > 65535) return fail(FHIP_E_ARGUMENT, "fhip_scale: more than 65535 columns in one call");
+  int64_t gx = cdiv64(rows, 256);
+  if (gx > 4096) gx = 4096;
+  const dim3 grid((unsigned)gx, (unsigned)cols);
+  hipStream_t s = (hipStream_t)stream;
+  switch (type) {
+    case FHIP_F32:
+      hipLaunchKernelGGL((scale_kernel<float, false>), grid, dim3(256), 0, s, (float *)Y, ldy, rows, (float)a_re,
+                         0.0f, 1);
+      break;
+    case FHIP_F64:
+      hipLaunchKernelGGL((scale_kernel<double, false>), grid, dim3(256), 0, s, (double *)Y, ldy, rows, a_re, 0.0, 1);
+      break;
+    case FHIP_C32:
+      hipLaunchKernelGGL((scale_kernel<float, true>), grid, dim3(256), 0, s, (float *)Y, ldy, rows, (float)a_re,
+                         (float)a_im, real_scalar);
+      break;
+    case FHIP_C64:
+      hipLaunchKernelGGL((scale_kernel<double, true>), grid, dim3(256), 0, s, (double *)Y, ldy, rows, a_re, a_im,
+                         real_scalar);
+      break;
+  }
+  return check_launch("scale_kernel");
+}
+
+extern "C" int fhip_dot_workspace_bytes(size_t *nbytes, int64_t rows, int64_t cols) {
+  if (!nbytes) return fail(FHIP_E_ARGUMENT, "fhip_dot_workspace_bytes: null output pointer");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_dot_workspace_bytes: negative dimension");
+  *nbytes = (size_t)(dot_chunks(rows) * cols) * 2 * sizeof(double);
+  return FHIP_OK;
+}
+
+extern "C" int fhip_dot(double *out, const void *X, int64_t ldx, const void *Y, int64_t ldy, int64_t rows,
+                        int64_t cols, int type, int mode, int accumulate, void *work, size_t work_bytes,
+                        void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_dot: unknown eltype");
+  if (mode != 0 && mode != 1) return fail(FHIP_E_ARGUMENT, "fhip_dot: mode must be 0 (conj(x) y) or 1 (|x|^2)");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_dot: negative dimension");
+  if (!out) return fail(FHIP_E_ARGUMENT, "fhip_dot: null output pointer");
+  if (cols > 65535) return fail(FHIP_E_ARGUMENT, "fhip_dot: more than 65535 columns in one call");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t nchunks = dot_chunks(rows);
+  const int64_t nslots = nchunks * cols;
+  if (nslots > 0) {
+    if (!X || (mode == 0 && !Y)) return fail(FHIP_E_ARGUMENT, "fhip_dot: null matrix pointer");
+    if (ldx < rows || (mode == 0 && ldy < rows))
+      return fail(FHIP_E_ARGUMENT, "fhip_dot: leading dimension smaller than the rows");
+    if (nchunks > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_dot: too many rows");
+    const size_t need = (size_t)nslots * 2 * sizeof(double);
+    if (!work || work_bytes < need)
+      return fail(FHIP_E_ARGUMENT, "fhip_dot: workspace of %zu bytes, %zu needed", work_bytes, need);
+    const dim3 grid((unsigned)nchunks, (unsigned)cols);
+    double *partial = (double *)work;
+    switch (type) {
+      case FHIP_F32:
+        hipLaunchKernelGGL((dot_partial_kernel<float, false>), grid, dim3(DOT_THREADS), 0, s, partial,
+                           (const float *)X, ldx, (const float *)Y, ldy, rows, nchunks, mode);
+        break;
+      case FHIP_F64:
+        hipLaunchKernelGGL((dot_partial_kernel<double, false>), grid, dim3(DOT_THREADS), 0, s, partial,
+                           (const double *)X, ldx, (const double *)Y, ldy, rows, nchunks, mode);
+        break;
+      case FHIP_C32:
+        hipLaunchKernelGGL((dot_partial_kernel<float, true>), grid, dim3(DOT_THREADS), 0, s, partial,
+                           (const float *)X, ldx, (const float *)Y, ldy, rows, nchunks, mode);
+        break;
+      case FHIP_C64:
+        hipLaunchKernelGGL((dot_partial_kernel<double, true>), grid, dim3(DOT_THREADS), 0, s, partial,
+                           (const double *)X, ldx, (const double *)Y, ldy, rows, nchunks, mode);
+        break;
+    }
+    const int st = check_launch("dot_partial_kernel");
+    if (st) return st;
+  }
+  hipLaunchKernelGGL(dot_finish_kernel, dim3(1), dim3(DOT_THREADS), 0, s, out, (const double *)work, nslots,
+                     accumulate);
+  return check_launch("dot_finish_kernel");
 }

@@ -28,21 +28,11 @@ struct GramShape;
 // (the 32 tile), otherwise the 4 waves take the four quadrants of the tile;
 // MB x NB is a wave's block count in 16x16 matrix-core tiles; RESIDENT is how
 // many such workgroups the MI355X holds at once (256 CUs x LDS/register limit).
-template <>
-struct GramShape<128> {
-  static constexpr int KC = 16;
-  static constexpr int KSPLIT = 1;
-  static constexpr int MB = 4, NB = 4;
-  static constexpr int RESIDENT = 512;
-};
 #ifndef FHIP_GRAM64_KC
 #define FHIP_GRAM64_KC 16
 #endif
 #ifndef FHIP_GRAM64_RESIDENT
 #define FHIP_GRAM64_RESIDENT 1024
-#endif
-#ifndef FHIP_GRAM_FP32_TILE
-#define FHIP_GRAM_FP32_TILE 64
 #endif
 #ifndef FHIP_GRAM_SWIZZLE
 #define FHIP_GRAM_SWIZZLE 1
@@ -101,13 +91,9 @@ constexpr int GRAM_MAX_Q = 256;
 
 static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm, bool fp32) {
   GramPlan p;
-  // FP32 products run at twice the FP64 rate, so the single-precision Gram
-  // takes 128-wide tiles to halve the operand traffic per flop.
-  p.tile = (ka > 64 && kb > 64 && fp32 && FHIP_GRAM_FP32_TILE == 128) ? 128 : (ka > 32 || kb > 32) ? 64 : 32;
+  p.tile = (ka > 32 || kb > 32) ? 64 : 32;
   int kc, resident;
-  if (p.tile == 128) {
-    kc = GramShape<128>::KC; p.ksplit = GramShape<128>::KSPLIT; resident = GramShape<128>::RESIDENT;
-  } else if (p.tile == 64) {
+  if (p.tile == 64) {
     kc = fp32 ? gram_kc<float, 64>() : gram_kc<double, 64>();
     p.ksplit = GramShape<64>::KSPLIT;
     resident = GramShape<64>::RESIDENT;
@@ -412,11 +398,6 @@ template <typename TIn, typename TC, typename TOut, bool CPLX>
 static int dispatch_gram(const GramPlan &p, void *C, int64_t ldc, const void *A, int64_t lda,
                          const void *B, int64_t ldb, int64_t m, int ka, int kb, bool herm, double ar,
                          double ai, double br, double bi, void *work, hipStream_t stream) {
-  if constexpr (sizeof(TC) == 4) {
-    if (p.tile == 128)
-      return launch_gram<TIn, TC, TOut, CPLX, 128>(p, (TOut *)C, ldc, (const TIn *)A, lda, (const TIn *)B,
-                                                   ldb, m, ka, kb, herm, ar, ai, br, bi, work, stream);
-  }
   if (p.tile == 64)
     return launch_gram<TIn, TC, TOut, CPLX, 64>(p, (TOut *)C, ldc, (const TIn *)A, lda, (const TIn *)B,
                                                 ldb, m, ka, kb, herm, ar, ai, br, bi, work, stream);

@@ -87,6 +87,9 @@ _SIGNATURES = {
     "fhip_fill": [_vp, _i, _i64, _i64, _i64, _d, _d, _vp],
     "fhip_rowdot": [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i, _i, _vp],
     "fhip_axpby": [_vp, _i64, _vp, _i64, _i64, _i64, _d, _d, _d, _d, _i, _vp],
+    "fhip_scale": [_vp, _i64, _i64, _i64, _d, _d, _i, _i, _vp],
+    "fhip_dot_workspace_bytes": [ctypes.POINTER(ctypes.c_size_t), _i64, _i64],
+    "fhip_dot": [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i, _i, _i, _vp, ctypes.c_size_t, _vp],
     "fhip_ghost_randn": [_vp, _i, _i64, _i64, _i64, _i64, _i64, _u64, _vp],
     "fhip_ghost_tables_checksum": [ctypes.POINTER(_u64)],
     "fhip_stencil_tridiag": [_vp, _i64, _vp, _i64, _i64, _i64, _d, _d, _d, _d, _vp, _i, _i, _vp],

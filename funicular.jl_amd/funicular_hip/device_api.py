@@ -545,6 +545,45 @@ class HIPBackend(DeviceBackend):
              eltype_code(Y.dtype), current_stream())
         return Y
 
+    def scale(self, Y, alpha):
+        """Y <- Y alpha entry by entry, as rmul! computes it: a real scalar
+        scales both parts, a complex one is the schoolbook product."""
+        real_scalar = not np.iscomplexobj(alpha)
+        a = complex(alpha)
+        if not np.issubdtype(Y.dtype, np.complexfloating) and a.imag != 0:
+            raise ArgumentError(f"cannot scale a real {Y.dtype} matrix by the complex number {alpha}")
+        code = eltype_code(Y.dtype)
+        for c0 in range(0, Y.cols, _MAX_GRID_COLS):
+            part = Y.view(None, (c0, min(Y.cols, c0 + _MAX_GRID_COLS)))
+            call("fhip_scale", part.ptr, part.ld, part.rows, part.cols, a.real, a.imag, 1 if real_scalar else 0, code,
+                 current_stream())
+        return Y
+
+    def dot_workspace_bytes(self, rows, cols):
+        n = ctypes.c_size_t()
+        call("fhip_dot_workspace_bytes", ctypes.byref(n), int(rows), int(min(cols, _MAX_GRID_COLS)))
+        return n.value
+
+    def dot_accumulate(self, out, X, Y=None, accumulate=False, work=None):
+        """out (a 1 x 1 complex128 device matrix) <- (accumulate ? out : 0) +
+        sum conj(X) .* Y, or sum |X|^2 when Y is None, summed in FP64 in an
+        order fixed by the shape."""
+        if Y is not None and (tuple(X.shape) != tuple(Y.shape) or X.dtype != Y.dtype):
+            raise DimensionMismatch(f"dot of a {X.shape} {X.dtype} and a {Y.shape} {Y.dtype} matrix")
+        if out.dtype != np.complex128 or out.size < 1:
+            raise ArgumentError("dot_accumulate needs a complex128 device accumulator")
+        work_ptr, work_bytes = _workspace(self, work, self.dot_workspace_bytes(X.rows, X.cols))
+        code = eltype_code(X.dtype)
+        mode = 1 if Y is None else 0
+        cols = max(X.cols, 1)
+        for c0 in range(0, cols, _MAX_GRID_COLS):
+            c1 = min(X.cols, c0 + _MAX_GRID_COLS)
+            xp = X.view(None, (c0, c1))
+            yp = xp if Y is None else Y.view(None, (c0, c1))
+            call("fhip_dot", out.ptr, xp.ptr, xp.ld, yp.ptr, yp.ld, xp.rows, xp.cols, code, mode,
+                 1 if (accumulate or c0 > 0) else 0, work_ptr, work_bytes, current_stream())
+        return out
+
     def rowdot(self, out, A, B, accumulate=False):
         """out[i] (+)= Re sum_j conj(A[i, j]) B[i, j]; out is a float64 device vector."""
         if tuple(A.shape) != tuple(B.shape) or A.dtype != B.dtype or out.rows < A.rows:
@@ -565,6 +604,10 @@ class HIPBackend(DeviceBackend):
         call("fhip_ghost_randn", dst.ptr, eltype_code(dst.dtype), dst.ld, int(row0), dst.rows, int(col0),
              dst.cols, int(seed) & 0xFFFFFFFFFFFFFFFF, current_stream())
         return dst
+
+
+# Columns per launch of the kernels that take one grid row per column.
+_MAX_GRID_COLS = 65535
 
 
 def _stream_of(queue):

@@ -1,5 +1,6 @@
-"""Panel BLAS: gram, panelmul, cholqr2, rightmul and project
-(src/panelblas.jl:1-301), on the sweep pipeline and the HIP kernels.
+"""Panel BLAS: gram, panelmul, cholqr2, rightmul, project and the
+elementwise scale / axpy / dot / norm (src/panelblas.jl:1-370), on the sweep
+pipeline and the HIP kernels.
 
 The step structure is the reference's: gram and the triangular solve sweep
 row blocks of height floor(N w / k); project runs p (p + 1) panel steps
@@ -284,6 +285,71 @@ def project(Q, G, *, nbuffers=None):
         return collect_small(accumulator, plan)
     finally:
         _plan.checkin_device_buffers(plan, [image, accumulator, work])
+
+
+def scale(Y, alpha, *, nbuffers=None):
+    """Multiplies every entry of Y by alpha in place, in one sweep
+    (src/panelblas.jl:303-316). Returns Y."""
+    backend = Y.plan.backend
+    sweep(lambda _, Yb: backend.scale(Yb, alpha), updatepanels(Y), nbuffers=nbuffers)
+    return Y
+
+
+def axpy(alpha, X, Y, *, nbuffers=None):
+    """Y <- alpha X + Y in one sweep (src/panelblas.jl:318-331). Returns Y."""
+    shared_plan(X, Y)
+    assert_conformal(X, Y)
+    if X is Y:
+        raise ArgumentError("axpy(alpha, X, X) doubles a matrix against itself. Use scale(X, 1 + alpha)")
+    if not np.issubdtype(Y.dtype, np.complexfloating) and complex(alpha).imag != 0:
+        raise ArgumentError(f"cannot add a complex multiple ({alpha}) of a real {Y.dtype} panel matrix")
+    backend = Y.plan.backend
+    sweep(lambda _, Xb, Yb: backend.axpby(Yb, Xb, alpha, 1.0), readpanels(X), updatepanels(Y), nbuffers=nbuffers)
+    return Y
+
+
+def _reduce(X, Y, nbuffers):
+    """Sum over every entry of conj(X) .* Y (|X|^2 when Y is None), one sweep
+    over column panels (src/panelblas.jl:333-370). Each step adds its panel's
+    sum, formed in FP64 in a fixed order, to a device scalar; the reference's
+    N x w accumulator and its host-side sum are not needed."""
+    plan = X.plan
+    backend = plan.backend
+    acc = _plan.checkout_device_buffers(plan, np.complex128, (1, 1), 1)[0]
+    work = None
+    try:
+        work = _plan.checkout_workspace(plan, backend.dot_workspace_bytes(X.N, X.w))
+        if Y is None:
+            sweep(lambda j, Xb: backend.dot_accumulate(acc, Xb, None, j > 0, work), readpanels(X),
+                  nbuffers=nbuffers)
+        else:
+            sweep(lambda j, Xb, Yb: backend.dot_accumulate(acc, Xb, Yb, j > 0, work), readpanels(X),
+                  readpanels(Y), nbuffers=nbuffers)
+        if npanels(X) == 0:
+            return 0j
+        return complex(collect_small(acc, plan)[0, 0])
+    finally:
+        _plan.checkin_device_buffers(plan, [acc, work])
+
+
+def dot(X, Y, *, nbuffers=None):
+    """sum(conj(X) .* Y) over every entry, in the eltype of X and Y
+    (src/panelblas.jl:333-354)."""
+    shared_plan(X, Y)
+    assert_conformal(X, Y)
+    v = _reduce(X, None if X is Y else Y, nbuffers)
+    T = X.dtype.type
+    return T(v) if np.issubdtype(X.dtype, np.complexfloating) else T(v.real)
+
+
+def norm(X, p=2, *, nbuffers=None):
+    """The Frobenius norm: the plain square root of the sum of |x|^2, without
+    the rescaling LinearAlgebra does for arrays (src/panelblas.jl:318-322,
+    356-370)."""
+    if p != 2:
+        raise ArgumentError(f"a PanelMatrix has only its Frobenius norm, p = 2, got p = {p}")
+    R = np.finfo(X.dtype).dtype.type
+    return R(np.sqrt(_reduce(X, None, nbuffers).real))
 
 
 UNIT_ROUNDOFF_FP32 = 2.0 ** -24

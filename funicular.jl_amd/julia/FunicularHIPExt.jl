@@ -188,6 +188,44 @@ function convert_copy!(::HIPBackend, dst::HIPMatrix{S}, src::HIPMatrix{T}) where
     dst
 end
 
+# scale! and axpy! run unchanged: their panel steps are rmul! and axpy! on the
+# staged device buffers (src/panelblas.jl:311-331).
+function LinearAlgebra.rmul!(Y::HIPMatrix{T}, α::Number) where {T}
+    a = ComplexF64(α)
+    check(ccall((:fhip_scale, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Int64, Float64, Float64, Cint, Cint, Ptr{Cvoid}),
+        Y.ptr, Y.ld, size(Y, 1), size(Y, 2), real(a), imag(a), Cint(α isa Real), eltcode(T), current_stream()))
+    Y
+end
+
+function LinearAlgebra.axpy!(α::Number, X::HIPMatrix{T}, Y::HIPMatrix{T}) where {T}
+    size(X) == size(Y) || throw(DimensionMismatch("axpy! between a $(size(X)) and a $(size(Y)) matrix"))
+    a = ComplexF64(α)
+    check(ccall((:fhip_axpby, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Float64, Float64, Float64, Float64, Cint, Ptr{Cvoid}),
+        Y.ptr, Y.ld, X.ptr, X.ld, size(Y, 1), size(Y, 2), real(a), imag(a), 1.0, 0.0, eltcode(T), current_stream()))
+    Y
+end
+
+# dot and norm (src/panelblas.jl:333-370) broadcast into an N x w device
+# accumulator, which a HIPMatrix cannot do. With a per-panel hook in Funicular,
+#
+#     panel_reduce!(b, acc, Xb, Yb, first) = <the broadcast step of dot/norm>
+#
+# (Yb === nothing for norm), this method sums the panel straight into a 1 x 1
+# ComplexF64 device scalar in FP64, in an order fixed by the panel's shape.
+function panel_reduce!(::HIPBackend, acc::HIPMatrix{ComplexF64}, Xb::HIPMatrix{T}, Yb, first::Bool) where {T}
+    n = Ref{Csize_t}(0)
+    check(ccall((:fhip_dot_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Int64), n, size(Xb, 1), size(Xb, 2)))
+    work = workspace(n[])
+    y = Yb === nothing ? Xb : Yb
+    check(ccall((:fhip_dot, LIB[]), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+        acc.ptr, Xb.ptr, Xb.ld, y.ptr, y.ld, size(Xb, 1), size(Xb, 2), eltcode(T), Cint(Yb === nothing),
+        Cint(!first), work.ptr, n[], current_stream()))
+    acc
+end
+
 # A per-backend workspace that grows to the largest request and is reused;
 # the first sweep of a shape allocates it, later ones do not.
 const WORK = Ref{Any}(nothing)

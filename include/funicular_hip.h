@@ -205,6 +205,27 @@ int fhip_convert_copy(void *dst, int dst_type, int64_t ldd, const void *src, int
 int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t rows, int64_t cols,
                double a_re, double a_im, double b_re, double b_im, int type, void *stream);
 
+/* Y <- Y alpha for a rows x cols matrix, entry by entry as rmul! computes it:
+ * a real scalar (real_scalar = 1, or a real eltype) scales both parts, a
+ * complex one is the schoolbook product, each part rounded once per product
+ * and sum (no fused multiply-add). At most 65535 columns per call.
+ * Replaces the rmul! step of scale! (src/panelblas.jl:311-316). */
+int fhip_scale(void *Y, int64_t ldy, int64_t rows, int64_t cols, double a_re, double a_im,
+               int real_scalar, int type, void *stream);
+
+/* Bytes of workspace fhip_dot needs for a rows x cols block. */
+int fhip_dot_workspace_bytes(size_t *nbytes, int64_t rows, int64_t cols);
+
+/* out[0..1] <- (accumulate ? out : 0) + sum over a rows x cols block of
+ * conj(X) .* Y (mode 0) or |X|^2 (mode 1, Y unused, imaginary part 0), summed
+ * in FP64 in an order fixed by (rows, cols) alone. out is a device pointer to
+ * (re, im) FP64; work holds fhip_dot_workspace_bytes. At most 65535 columns.
+ * Replaces the per-panel broadcasts of dot and norm and their final sum
+ * (src/panelblas.jl:333-370), without the N x w accumulator. */
+int fhip_dot(double *out, const void *X, int64_t ldx, const void *Y, int64_t ldy, int64_t rows,
+             int64_t cols, int type, int mode, int accumulate, void *work, size_t work_bytes,
+             void *stream);
+
 /* out[i] (+)= Re sum_j conj(A[i, j]) B[i, j] for i < rows (out is FP64,
  * overwritten unless accumulate = 1): the per-row reduction behind the
  * sensitivity of S = Q^H G Q to the operator's diagonal, dL/dd_i =

@@ -111,6 +111,64 @@ def panelmul(apply, X, w):
     return out
 
 
+def scale(Y, alpha):
+    """src/panelblas.jl:311-316 (rmul! on each panel): Julia's Complex * Real
+    scales both parts, Complex * Complex is the schoolbook product, each part
+    rounded once per product and sum. Spelled out on the real parts so that
+    numpy's complex loops (which may fuse) play no part."""
+    Y = np.asarray(Y)
+    if not np.iscomplexobj(Y):
+        return Y * Y.dtype.type(np.real(alpha))
+    R = Y.real.dtype.type
+    yr, yi = Y.real.copy(), Y.imag.copy()
+    with np.errstate(invalid="ignore", over="ignore"):
+        if not np.iscomplexobj(alpha):
+            a = R(alpha)
+            re, im = yr * a, yi * a
+        else:
+            ar, ai = R(alpha.real), R(alpha.imag)
+            re = yr * ar - yi * ai
+            im = yr * ai + yi * ar
+    out = np.empty_like(Y)
+    out.real, out.imag = re, im
+    return out
+
+
+def axpy(alpha, X, Y):
+    """src/panelblas.jl:323-331: Y + alpha X, panel by panel (the panels are
+    independent, so the order does not matter)."""
+    return np.asarray(Y) + np.asarray(alpha, dtype=np.asarray(X).dtype) * np.asarray(X)
+
+
+def dot(A, B, w):
+    """src/panelblas.jl:333-354: an N x w accumulator that the first panel
+    overwrites with conj(A_j) .* B_j and later panels add to (columns
+    1:width of each panel), then the sum of the accumulator."""
+    N, k = A.shape
+    acc = None
+    for (c0, c1) in P.panel_ranges(k, w):
+        prod = A[:, c0:c1].conj() * B[:, c0:c1]
+        if acc is None:
+            acc = prod.copy()
+        else:
+            acc[:, :c1 - c0] += prod
+    return acc.sum() if acc is not None else A.dtype.type(0)
+
+
+def norm(A, w):
+    """src/panelblas.jl:356-370: the square root of the summed |a|^2
+    accumulator, with no rescaling."""
+    N, k = A.shape
+    acc = None
+    for (c0, c1) in P.panel_ranges(k, w):
+        sq = np.abs(A[:, c0:c1]) ** 2
+        if acc is None:
+            acc = sq.copy()
+        else:
+            acc[:, :c1 - c0] += sq
+    return np.sqrt(acc.sum()) if acc is not None else 0.0
+
+
 def orthogonality_error(Q):
     """||Q^H Q - I||_F."""
     k = Q.shape[1]

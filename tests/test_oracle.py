@@ -115,6 +115,59 @@ def test_project_against_explicit_product():
         assert not np.isnan(S).any()  # every block assigned exactly once
 
 
+def test_scale_known_answers():
+    # (1 + 2i)(3 - i) = 5 + 5i exactly; real scalars scale each part.
+    assert OB.scale(np.array([1 + 2j]), 3 - 1j)[0] == 5 + 5j
+    assert OB.scale(np.array([1.5 - 2j]), 2.0)[0] == 3 - 4j
+    assert OB.scale(np.array([1.5, -2.0]), 2.0).tolist() == [3.0, -4.0]
+    # Julia's Complex * Real never forms inf * 0, Complex * Complex does: a
+    # dropped distinction between the two shows up here.
+    y = np.array([complex(np.inf, 0.0)])
+    assert OB.scale(y, 2.0)[0] == complex(np.inf, 0.0)
+    z = OB.scale(y, 2.0 + 0j)[0]
+    assert z.real == np.inf and np.isnan(z.imag)
+    # Each part is the rounded sum of two rounded products (no fused step).
+    rng = np.random.default_rng(11)
+    Y = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    a = 0.3 - 1.7j
+    got = OB.scale(Y, a)
+    for yv, gv in zip(Y, got):
+        re = float(np.float64(yv.real * a.real) - np.float64(yv.imag * a.imag))
+        im = float(np.float64(yv.real * a.imag) + np.float64(yv.imag * a.real))
+        assert gv.real == re and gv.imag == im
+
+
+def test_dot_and_norm_match_exact_sums():
+    rng = np.random.default_rng(12)
+    N, k = 37, 11
+    A = rng.standard_normal((N, k)) + 1j * rng.standard_normal((N, k))
+    B = rng.standard_normal((N, k)) + 1j * rng.standard_normal((N, k))
+    # Correctly rounded sums of the exact products (conj on the first operand).
+    mp = mpmath.mp
+    mp.prec = 200
+    exact = mp.mpc(0)
+    for a_, b_ in zip(A.ravel(), B.ravel()):
+        exact += mp.conj(mp.mpc(a_.real, a_.imag)) * mp.mpc(b_.real, b_.imag)
+    ref = complex(exact)
+    sq = sum(mp.mpf(a_.real) ** 2 + mp.mpf(a_.imag) ** 2 for a_ in A.ravel())
+    for w in (1, 4, 11):
+        d = OB.dot(A, B, w)
+        assert abs(d - ref) <= 1e-13 * abs(ref)
+        assert abs(OB.norm(A, w) - float(mp.sqrt(sq))) <= 1e-14 * float(mp.sqrt(sq))
+    # conj goes on the first operand: dot(A, iA) = i |A|^2.
+    assert abs(OB.dot(A, 1j * A, 4) - 1j * float(sq)) <= 1e-13 * float(sq)
+    assert OB.norm(np.array([[3.0, 4.0]]), 1) == 5.0
+    # Real eltypes: a plain sum of products.
+    Ar = A.real.copy()
+    assert abs(OB.dot(Ar, Ar, 3) - float(np.sum(Ar * Ar))) <= 1e-13 * float(np.sum(Ar * Ar))
+
+
+def test_axpy_is_alpha_x_plus_y():
+    X = np.array([1 + 1j, 2 - 1j])
+    Y = np.array([0.5j, -1.0])
+    assert OB.axpy(2 - 1j, X, Y).tolist() == [(2 - 1j) * (1 + 1j) + 0.5j, (2 - 1j) * (2 - 1j) - 1.0]
+
+
 def test_tridiagonal_against_dense():
     rng = np.random.default_rng(8)
     N = 17
