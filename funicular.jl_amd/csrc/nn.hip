@@ -1,6 +1,8 @@
 // Kernel (b'): the row-block product D <- A C that replaces rightmul_gemm!
 // (src/kernels.jl:73, semantics :57-64, called from rightmul!
-// src/panelblas.jl:203-245), the panel update of the QR adjoint.
+// src/panelblas.jl:203-245), the panel update of the QR adjoint; and its
+// scaled form D <- alpha A C + beta D, the update step of the panel-by-panel
+// (left-looking) triangular solve.
 //
 // Rows are independent: a workgroup owns GW_ROWS rows and GW_CB output
 // columns, its four waves a 32 x 32 tile each. The inner dimension streams
@@ -21,10 +23,11 @@ constexpr int GW_KC = 16;    // inner dimension per LDS slice
 constexpr int GW_APT = GW_KC * GW_ROWS / GW_THREADS;  // A elements per thread per slice
 constexpr int GW_BPT = GW_KC * GW_CB / GW_THREADS;    // C elements per thread per slice
 
-template <typename T, bool CPLX>
+// SCALED: D <- alpha A C + beta D (beta = 0 does not read D); otherwise D <- A C.
+template <typename T, bool CPLX, bool SCALED>
 __global__ __launch_bounds__(GW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void gemm_nn_kernel(
     T *__restrict__ D, int64_t ldd, const T *__restrict__ A, int64_t lda, const T *__restrict__ C, int64_t ldc,
-    int64_t m, int s, int r, int ncb, int64_t nrc) {
+    int64_t m, int s, int r, int ncb, int64_t nrc, T alr, T ali, T ber, T bei) {
   constexpr int NP = CPLX ? 2 : 1;
   // Complex FP64 planes use the swizzled LDS layout (laid out for 8-byte
   // elements); the others pad each row by 4.
@@ -128,21 +131,48 @@ __global__ __launch_bounds__(GW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       for (int q = 0; q < 4; ++q) {
         const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, q);
         const int col = c0 + nj0 + 16 * nb + li;
-        if (row < m && col < r) store_elem<T, CPLX>(D, row + (int64_t)col * ldd, re[q], im[q]);
+        if (row < m && col < r) {
+          if constexpr (SCALED) {
+            T outr = alr * re[q] - ali * im[q];
+            T outi = alr * im[q] + ali * re[q];
+            if (ber != T(0) || bei != T(0)) {
+              T dr, di;
+              load_elem<T, T, CPLX>(D, row + (int64_t)col * ldd, true, dr, di);
+              outr += ber * dr - bei * di;
+              outi += ber * di + bei * dr;
+            }
+            store_elem<T, CPLX>(D, row + (int64_t)col * ldd, outr, outi);
+          } else {
+            store_elem<T, CPLX>(D, row + (int64_t)col * ldd, re[q], im[q]);
+          }
+        }
       }
     }
 }
 
-template <typename T, bool CPLX>
+template <typename T, bool CPLX, bool SCALED>
 static int launch_gemm(T *D, int64_t ldd, const T *A, int64_t lda, const T *C, int64_t ldc, int64_t m,
-                       int s, int r, hipStream_t stream) {
+                       int s, int r, hipStream_t stream, double alr = 1, double ali = 0, double ber = 0,
+                       double bei = 0) {
   const int ncb = (r + GW_CB - 1) / GW_CB;
   const int64_t nrc = cdiv64(m, GW_ROWS);
   const int64_t grid = 8 * cdiv64(nrc, 8) * ncb;
   if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_gemm_nn: too many rows (%lld)", (long long)m);
-  hipLaunchKernelGGL((gemm_nn_kernel<T, CPLX>), dim3((unsigned)grid), dim3(GW_THREADS), 0, stream, D, ldd, A, lda,
-                     C, ldc, m, s, r, ncb, nrc);
+  hipLaunchKernelGGL((gemm_nn_kernel<T, CPLX, SCALED>), dim3((unsigned)grid), dim3(GW_THREADS), 0, stream, D, ldd, A,
+                     lda, C, ldc, m, s, r, ncb, nrc, (T)alr, (T)ali, (T)ber, (T)bei);
   return check_launch("gemm_nn_kernel");
+}
+
+static int gemm_checks(const char *name, void *D, int64_t ldd, const void *A, int64_t lda, const void *C,
+                       int64_t ldc, int64_t m, int64_t s, int64_t r, int type) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "%s: unknown type %d", name, type);
+  if (m < 0 || s < 0 || r < 0) return fail(FHIP_E_ARGUMENT, "%s: negative dimension", name);
+  if (s > (1 << 20) || r > (1 << 20)) return fail(FHIP_E_ARGUMENT, "%s: too many columns", name);
+  if (m == 0 || r == 0) return FHIP_OK;
+  if (!D || (s > 0 && (!A || !C))) return fail(FHIP_E_ARGUMENT, "%s: null matrix pointer", name);
+  if (ldd < m || (s > 0 && (lda < m || ldc < s)))
+    return fail(FHIP_E_ARGUMENT, "%s: leading dimension smaller than the matrix", name);
+  return FHIP_OK;
 }
 
 }  // namespace fhip
@@ -151,20 +181,35 @@ using namespace fhip;
 
 extern "C" int fhip_gemm_nn(void *D, int64_t ldd, const void *A, int64_t lda, const void *C, int64_t ldc,
                             int64_t m, int64_t s, int64_t r, int type, void *stream) {
-  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_gemm_nn: unknown type %d", type);
-  if (m < 0 || s < 0 || r < 0) return fail(FHIP_E_ARGUMENT, "fhip_gemm_nn: negative dimension");
+  int st = gemm_checks("fhip_gemm_nn", D, ldd, A, lda, C, ldc, m, s, r, type);
+  if (st) return st;
   if (m == 0 || r == 0) return FHIP_OK;
-  if (s > (1 << 20) || r > (1 << 20)) return fail(FHIP_E_ARGUMENT, "fhip_gemm_nn: too many columns");
-  if (!D || (s > 0 && (!A || !C))) return fail(FHIP_E_ARGUMENT, "fhip_gemm_nn: null matrix pointer");
-  if (ldd < m || (s > 0 && (lda < m || ldc < s)))
-    return fail(FHIP_E_ARGUMENT, "fhip_gemm_nn: leading dimension smaller than the matrix");
-  hipStream_t st = (hipStream_t)stream;
+  hipStream_t sm = (hipStream_t)stream;
   const int is = (int)s, ir = (int)r;
   switch (type) {
-    case FHIP_C64: return launch_gemm<double, true>((double *)D, ldd, (const double *)A, lda, (const double *)C, ldc, m, is, ir, st);
-    case FHIP_C32: return launch_gemm<float, true>((float *)D, ldd, (const float *)A, lda, (const float *)C, ldc, m, is, ir, st);
-    case FHIP_F64: return launch_gemm<double, false>((double *)D, ldd, (const double *)A, lda, (const double *)C, ldc, m, is, ir, st);
-    case FHIP_F32: return launch_gemm<float, false>((float *)D, ldd, (const float *)A, lda, (const float *)C, ldc, m, is, ir, st);
+    case FHIP_C64: return launch_gemm<double, true, false>((double *)D, ldd, (const double *)A, lda, (const double *)C, ldc, m, is, ir, sm);
+    case FHIP_C32: return launch_gemm<float, true, false>((float *)D, ldd, (const float *)A, lda, (const float *)C, ldc, m, is, ir, sm);
+    case FHIP_F64: return launch_gemm<double, false, false>((double *)D, ldd, (const double *)A, lda, (const double *)C, ldc, m, is, ir, sm);
+    case FHIP_F32: return launch_gemm<float, false, false>((float *)D, ldd, (const float *)A, lda, (const float *)C, ldc, m, is, ir, sm);
   }
   return fail(FHIP_E_UNSUPPORTED, "fhip_gemm_nn: unknown type %d", type);
+}
+
+extern "C" int fhip_gemm_nn_scaled(void *D, int64_t ldd, const void *A, int64_t lda, const void *C, int64_t ldc,
+                                   int64_t m, int64_t s, int64_t r, double alpha_re, double alpha_im,
+                                   double beta_re, double beta_im, int type, void *stream) {
+  int st = gemm_checks("fhip_gemm_nn_scaled", D, ldd, A, lda, C, ldc, m, s, r, type);
+  if (st) return st;
+  if (m == 0 || r == 0) return FHIP_OK;
+  hipStream_t sm = (hipStream_t)stream;
+  const int is = (int)s, ir = (int)r;
+  const bool cplx = type == FHIP_C64 || type == FHIP_C32;
+  const double ai = cplx ? alpha_im : 0.0, bi = cplx ? beta_im : 0.0;
+  switch (type) {
+    case FHIP_C64: return launch_gemm<double, true, true>((double *)D, ldd, (const double *)A, lda, (const double *)C, ldc, m, is, ir, sm, alpha_re, ai, beta_re, bi);
+    case FHIP_C32: return launch_gemm<float, true, true>((float *)D, ldd, (const float *)A, lda, (const float *)C, ldc, m, is, ir, sm, alpha_re, ai, beta_re, bi);
+    case FHIP_F64: return launch_gemm<double, false, true>((double *)D, ldd, (const double *)A, lda, (const double *)C, ldc, m, is, ir, sm, alpha_re, ai, beta_re, bi);
+    case FHIP_F32: return launch_gemm<float, false, true>((float *)D, ldd, (const float *)A, lda, (const float *)C, ldc, m, is, ir, sm, alpha_re, ai, beta_re, bi);
+  }
+  return fail(FHIP_E_UNSUPPORTED, "fhip_gemm_nn_scaled: unknown type %d", type);
 }

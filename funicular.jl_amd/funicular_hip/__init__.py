@@ -20,8 +20,8 @@ from .ghosts import GhostPanels  # noqa: E402
 from .io import disk_reads, disk_writes, load, save  # noqa: E402
 from .operator import (DenseOperator, Helmholtz3D, Tridiagonal1D, check_operator, ishermitian_op,  # noqa: E402
                        panel_capable, workspace_bytes)
-from .panelblas import (axpy, cholesky_upper, cholqr2, cholqr2_mixed, dot, gram, norm, panelmul,  # noqa: E402
-                        project, rdiv_rows, rightmul, scale)
+from .panelblas import (axpy, cholesky_upper, cholqr2, cholqr2_mixed, cholqr2_panelwise, dot, gram,  # noqa: E402
+                        gram_panelwise, norm, panelmul, project, rdiv_panelwise, rdiv_rows, rightmul, scale)
 from .panels import (PanelMatrix, copycols, copyto, evict, free, npanels, panelrange, panelwidth,  # noqa: E402
                      prefetch, similar, to_matrix)
 from .plan import (ResidencyPlan, choose_panel_width, device_bytes_allocated, resolve_panel_width,  # noqa: E402
@@ -31,4 +31,5 @@ __all__ = [
     "GhostPanels", "PanelMatrix", "ResidencyPlan", "check_operator", "cholqr2", "copycols", "foreachpanel",
     "gram", "npanels", "panelmul", "panelrange", "panelwidth", "project", "rightmul", "hip_backend",
     "rrule_cholqr2", "rrule_project", "save", "load", "disk_reads", "disk_writes", "scale", "axpy", "dot", "norm",
+    "cholqr2_panelwise", "gram_panelwise", "rdiv_panelwise",
 ]

@@ -499,6 +499,19 @@ class HIPBackend(DeviceBackend):
              eltype_code(dst.dtype), current_stream())
         return dst
 
+    def rightmul_gemm_scaled(self, dst, A, C, alpha, beta):
+        """dst <- alpha A C + beta dst (beta = 0 does not read dst)."""
+        if not (A.cols == C.rows and dst.shape == (A.rows, C.cols)):
+            raise DimensionMismatch(f"cannot multiply a {A.shape} block by a {C.shape} factor into a {dst.shape} matrix")
+        if not (A.dtype == C.dtype == dst.dtype):
+            raise ArgumentError("rightmul_gemm_scaled needs one eltype for A, C and dst")
+        if dst.size == 0:
+            return dst
+        a, b = complex(alpha), complex(beta)
+        call("fhip_gemm_nn_scaled", dst.ptr, dst.ld, A.ptr, A.ld, C.ptr, C.ld, A.rows, A.cols, C.cols,
+             a.real, a.imag, b.real, b.imag, eltype_code(dst.dtype), current_stream())
+        return dst
+
     def rdiv_upper(self, Y, R, work=None, compute=None):
         """Y <- Y R^-1 for upper triangular R (src/kernels.jl:108)."""
         if not (R.rows == R.cols == Y.cols):

@@ -287,6 +287,118 @@ def project(Q, G, *, nbuffers=None):
         _plan.checkin_device_buffers(plan, [image, accumulator, work])
 
 
+# CholeskyQR by column panels, for a basis that the host budget cannot hold
+# whole. A row sweep stages every panel at once (src/executor.jl:66-67), so
+# gram and cholqr2 refuse such a matrix; these sweeps hold one panel on the
+# device while the others stream past it, so only the staging buffers need to
+# fit. The reference has no such traversal.
+
+def panel_schedule_gram(p):
+    """Steps of gram_panelwise: for each panel j, (True, j, j) stages panel j
+    and forms its diagonal block; (False, i, j), i < j, then forms block (i, j)
+    against the held copy of panel j. p (p + 1) / 2 steps."""
+    panels, labels = [], []
+    for j in range(p):
+        panels.append(j)
+        labels.append((True, j, j))
+        for i in range(j):
+            panels.append(i)
+            labels.append((False, i, j))
+    return panels, labels
+
+
+def gram_panelwise(X, *, nbuffers=None):
+    """The Hermitian k x k Gram X^H X from pairs of column panels: block (i, j)
+    of the upper triangle is X_i^H X_j, formed with panel j held on the device.
+    The lower triangle is mirrored on the host."""
+    plan = X.plan
+    backend = plan.backend
+    T = X.dtype
+    p = npanels(X)
+    panels, labels = panel_schedule_gram(p)
+    hold = _plan.checkout_device_buffers(plan, T, (X.N, X.w), 1)[0]
+    acc = work = None
+    try:
+        acc = _plan.checkout_device_buffers(plan, T, (X.k, X.k), 1)[0]
+        need = max(backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=True),
+                   backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=False))
+        work = _plan.checkout_workspace(plan, need)
+
+        def step(label, Xb):
+            first, i, j = label
+            rj = panelrange(X, j)
+            held = hold.view(None, (0, panelwidth(X, j)))
+            if first:
+                backend.convert_copy(held, Xb)
+                backend.gram_accumulate(acc.view((rj.start, rj.stop), (rj.start, rj.stop)), Xb, Xb, 1.0, 0.0, work)
+            else:
+                ri = panelrange(X, i)
+                backend.gram_accumulate(acc.view((ri.start, ri.stop), (rj.start, rj.stop)), Xb, held, 1.0, 0.0, work)
+        sweep(step, readpanels(X), nbuffers=nbuffers, traversal=PanelSteps(panels, labels))
+        G = collect_small(acc, plan)
+    finally:
+        _plan.checkin_device_buffers(plan, [hold, acc, work])
+    return np.triu(G) + np.triu(G, 1).conj().T
+
+
+def rdiv_panelwise(Y, R, *, nbuffers=None):
+    """Y <- Y R^-1 column panel by column panel, left looking:
+    Y_j <- (Y_j - sum_{i<j} Y_i R_ij) R_jj^-1, the panels before j already
+    solved. Panel j is held on the device while the solved panels stream past
+    it, then solved and written back."""
+    plan = Y.plan
+    backend = plan.backend
+    T = Y.dtype
+    p = npanels(Y)
+    hold = _plan.checkout_device_buffers(plan, T, (Y.N, Y.w), 1)[0]
+    factor = work = None
+    try:
+        factor = _plan.checkout_device_buffers(plan, T, R.shape, 1)[0]
+        backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
+        work = _plan.checkout_workspace(plan, backend.trsm_workspace_bytes(Y.w, T))
+        for j in range(p):
+            rj = panelrange(Y, j)
+            held = hold.view(None, (0, panelwidth(Y, j)))
+
+            def update(label, Yb, rj=rj, held=held):
+                first, i = label
+                if first:
+                    backend.convert_copy(held, Yb)
+                else:
+                    ri = panelrange(Y, i)
+                    backend.rightmul_gemm_scaled(held, Yb, factor.view((ri.start, ri.stop), (rj.start, rj.stop)),
+                                                 -1.0, 1.0)
+            sweep(update, readpanels(Y), nbuffers=nbuffers,
+                  traversal=PanelSteps([j] + list(range(j)), [(True, j)] + [(False, i) for i in range(j)]))
+
+            def solve(_, Yb, rj=rj, held=held):
+                backend.rdiv_upper(held, factor.view((rj.start, rj.stop), (rj.start, rj.stop)), work)
+                backend.convert_copy(Yb, held)
+            sweep(solve, writepanels(Y), nbuffers=1, traversal=PanelSteps([j]))
+    finally:
+        _plan.checkin_device_buffers(plan, [hold, factor, work])
+    return Y
+
+
+def cholqr2_panelwise(Y, *, nbuffers=None):
+    """cholqr2 by column panels (gram_panelwise, host Cholesky,
+    rdiv_panelwise), for a basis whose panels do not all fit the host budget.
+    Two passes and a third when a Cholesky needed a shift, as cholqr2.
+    Returns R with Q R equal to the original Y; Y holds Q."""
+    def one_pass():
+        G = gram_panelwise(Y, nbuffers=nbuffers)
+        R, shifted = cholesky_upper(G, Y.N, Y.k)
+        rdiv_panelwise(Y, R, nbuffers=nbuffers)
+        return R, shifted
+    R, shifted = one_pass()
+    R2, shifted2 = one_pass()
+    R = triangular_product(R2, R)
+    if shifted or shifted2:
+        R3, _ = one_pass()
+        R = triangular_product(R3, R)
+    return R
+
+
 def scale(Y, alpha, *, nbuffers=None):
     """Multiplies every entry of Y by alpha in place, in one sweep
     (src/panelblas.jl:303-316). Returns Y."""

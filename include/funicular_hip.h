@@ -194,6 +194,16 @@ int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, in
 int fhip_gemm_nn(void *D, int64_t ldd, const void *A, int64_t lda, const void *C,
                  int64_t ldc, int64_t m, int64_t s, int64_t r, int type, void *stream);
 
+/* D <- alpha A C + beta D, shapes as fhip_gemm_nn (beta = 0 does not read D;
+ * the imaginary parts are ignored for real types). The update step
+ * Y_j <- Y_j - Q_i R_ij of the panel-by-panel triangular solve that
+ * cholqr2 of a basis larger than the host budget needs; the reference has no
+ * such traversal (its row sweeps need every panel host resident,
+ * src/executor.jl:66-67). */
+int fhip_gemm_nn_scaled(void *D, int64_t ldd, const void *A, int64_t lda, const void *C,
+                        int64_t ldc, int64_t m, int64_t s, int64_t r, double alpha_re,
+                        double alpha_im, double beta_re, double beta_im, int type, void *stream);
+
 /* dst <- convert(dst_type, src), both rows x cols on the device.
  * Replaces convert_copy! (src/kernels.jl:14-20). */
 int fhip_convert_copy(void *dst, int dst_type, int64_t ldd, const void *src, int src_type,

@@ -111,6 +111,49 @@ def panelmul(apply, X, w):
     return out
 
 
+def gram_panelwise(A, w):
+    """X^H X from column panel pairs: block (i, j) of the upper triangle is
+    A_i^H A_j, the lower triangle mirrored. Not in the reference (its gram is
+    a row sweep); restated from the block identity (A^H A)_ij = A_i^H A_j."""
+    N, k = A.shape
+    G = np.zeros((k, k), dtype=A.dtype)
+    ranges = P.panel_ranges(k, w)
+    for j, (c0, c1) in enumerate(ranges):
+        for (d0, d1) in ranges[:j + 1]:
+            G[d0:d1, c0:c1] = A[:, d0:d1].conj().T @ A[:, c0:c1]
+    return np.triu(G) + np.triu(G, 1).conj().T
+
+
+def rdiv_panelwise(Y, R, w):
+    """Y R^-1 by column panels, left looking: X_j = (Y_j - sum_{i<j} X_i R_ij)
+    R_jj^-1. Not in the reference; restated from block back substitution."""
+    X = np.array(Y, copy=True)
+    ranges = P.panel_ranges(Y.shape[1], w)
+    for j, (c0, c1) in enumerate(ranges):
+        H = np.array(Y[:, c0:c1], copy=True)
+        for (d0, d1) in ranges[:j]:
+            H -= X[:, d0:d1] @ R[d0:d1, c0:c1]
+        X[:, c0:c1] = rdiv_upper(H, R[c0:c1, c0:c1])
+    return X
+
+
+def cholqr2_panelwise(Y, w):
+    """cholqr2 with gram_panelwise and rdiv_panelwise in place of the row
+    sweeps (two passes, a third after a shift). Returns (Q, R)."""
+    N, k = Y.shape
+    Q = np.array(Y, copy=True)
+    R_total = None
+    shifted_any = False
+    for npass in range(3):
+        if npass == 2 and not shifted_any:
+            break
+        R, shifted = cholesky_upper(gram_panelwise(Q, w), N, k)
+        shifted_any = shifted_any or shifted
+        Q = rdiv_panelwise(Q, R, w)
+        R_total = R if R_total is None else R @ R_total
+    return Q, R_total
+
+
 def scale(Y, alpha):
     """src/panelblas.jl:311-316 (rmul! on each panel): Julia's Complex * Real
     scales both parts, Complex * Complex is the schoolbook product, each part

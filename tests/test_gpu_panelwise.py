@@ -1,0 +1,128 @@
+"""cholqr2 by column panels (gram_panelwise, rdiv_panelwise,
+cholqr2_panelwise) on the HIP backend against the oracle, including a
+disk-backed basis larger than the host budget, which the row sweeps of
+gram and cholqr2 refuse (test/test_disk.jl:224-234)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import funicular_hip as F
+from funicular_hip import plan as P
+from oracle import ghost as OG
+from oracle import panelblas as OB
+
+pytestmark = pytest.mark.gpu
+
+WIDTHS = (23, 12, 7, 1)
+C16 = np.complex128
+
+
+def randmatrix(rng, N, k, T=C16):
+    A = rng.random((N, k)) - 0.5
+    if np.issubdtype(T, np.complexfloating):
+        A = A + 1j * (rng.random((N, k)) - 0.5)
+    return np.asfortranarray(A.astype(T))
+
+
+def make_plan(backend, **kw):
+    kw.setdefault("device_budget", 64 * 2 ** 20)
+    kw.setdefault("host_budget", 64 * 2 ** 20)
+    return F.ResidencyPlan(backend=backend, **kw)
+
+
+@pytest.mark.parametrize("T", [C16, np.complex64, np.float64])
+@pytest.mark.parametrize("w", WIDTHS)
+@pytest.mark.parametrize("nb", (1, 2))
+def test_panelwise_against_the_oracle(backend, T, w, nb):
+    print(f"panelwise T={np.dtype(T).name} w={w} nb={nb}", flush=True)
+    rng = np.random.default_rng(10 * w + nb)
+    N, k = 40, 23
+    A = randmatrix(rng, N, k, T)
+    plan = make_plan(backend)
+    tol = OB.blastol(T, N, k)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    assert OB.isapprox(F.gram_panelwise(X, nbuffers=nb), OB.gram_panelwise(A, w), tol)
+    R, _ = OB.cholesky_upper(OB.gram(A), N, k)
+    F.rdiv_panelwise(X, R, nbuffers=nb)
+    assert OB.isapprox(F.to_matrix(X), OB.rdiv_panelwise(A, R, w), tol)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Rq = F.cholqr2_panelwise(Y, nbuffers=nb)
+    Qo, Ro = OB.cholqr2_panelwise(A, w)
+    assert OB.isapprox(Rq, Ro, tol)
+    Q = F.to_matrix(Y)
+    assert OB.orthogonality_error(Q) <= OB.orthogonality_bound(T, N, k)
+    assert np.linalg.norm(Q @ Rq - A) <= tol * np.linalg.norm(A)
+
+
+def test_panelwise_on_a_device_resident_plan(backend):
+    rng = np.random.default_rng(3)
+    N, k, w = 3000, 40, 8
+    A = randmatrix(rng, N, k)
+    plan = F.ResidencyPlan(backend=backend, device_budget=1 << 30, host_budget=1 << 30, device_resident=True)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    R = F.cholqr2_panelwise(Y)
+    Qo, Ro = OB.cholqr2(A, w)
+    tol = OB.blastol(C16, N, k)
+    assert OB.isapprox(R, Ro, tol)
+    assert OB.isapprox(F.to_matrix(Y), Qo, tol)
+
+
+def test_panelwise_survives_the_shifted_third_pass(backend):
+    rng = np.random.default_rng(5)
+    N, k, w = 400, 24, 5
+    U = scipy.linalg.qr(randmatrix(rng, N, k), mode="economic")[0]
+    V = scipy.linalg.qr(randmatrix(rng, k, k))[0]
+    A = np.asfortranarray(U @ np.diag(10.0 ** np.linspace(0, -10, k)) @ V.conj().T)
+    plan = make_plan(backend)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    R = F.cholqr2_panelwise(Y)
+    Q = F.to_matrix(Y)
+    assert OB.orthogonality_error(Q) <= 1e-8
+    assert np.linalg.norm(Q @ R - A) <= 1e-10 * np.linalg.norm(A)
+
+
+def test_panelwise_cholqr2_of_a_basis_past_the_host_budget(backend, tmp_path):
+    # Eight panels on disk behind a host budget of four: the row sweeps need
+    # all eight host resident at once and refuse; the panel-wise passes only
+    # ever stage a few.
+    print("panelwise disk-backed", flush=True)
+    rng = np.random.default_rng(9)
+    N, k, w = 5000, 32, 4
+    A = randmatrix(rng, N, k)
+    budget = 4 * P.align_up(P.panel_bytes(N, w, np.dtype(C16)), P.SLAB_ALIGN)
+    plan = F.ResidencyPlan(backend=backend, device_budget=64 << 20, host_budget=budget,
+                           scratch_dir=str(tmp_path), panel_width=w)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    with pytest.raises(F.ArgumentError):
+        F.gram(Y)
+    assert all(p.pins == 0 for p in Y.panels)
+    assert np.array_equal(F.to_matrix(Y), A)
+    before = F.disk_reads(Y)
+    R = F.cholqr2_panelwise(Y)
+    assert F.disk_reads(Y) > before and F.disk_writes(Y) > 0
+    assert all(p.pins == 0 for p in Y.panels)
+    Qo, Ro = OB.cholqr2(A, w)
+    tol = OB.blastol(C16, N, k)
+    assert OB.isapprox(R, Ro, tol)
+    assert OB.isapprox(F.to_matrix(Y), Qo, tol)
+
+
+def test_panelwise_matches_the_row_sweeps_on_a_ghost_basis(backend):
+    # N = 2^20: the two traversals sum in different orders, so they agree to
+    # rounding, and both meet the orthogonality target.
+    N, k, w = 1 << 20, 128, 16
+    print(f"panelwise ghost N={N} k={k}", flush=True)
+    plan = F.ResidencyPlan(backend=backend, device_budget=32 << 30, host_budget=32 << 30)
+    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    Y1 = F.PanelMatrix(N, k, plan=plan, w=w)
+    Y2 = F.PanelMatrix(N, k, plan=plan, w=w)
+    F.copyto(Y1, O)
+    F.copyto(Y2, O)
+    R1 = F.cholqr2(Y1)
+    R2 = F.cholqr2_panelwise(Y2)
+    assert np.linalg.norm(R1 - R2) <= 1e-12 * np.linalg.norm(R1)
+    for Y in (Y1, Y2):
+        G = F.gram(Y)
+        assert np.linalg.norm(G - np.eye(k)) <= 1e-13
+    for Y in (Y1, Y2):
+        F.free(Y)

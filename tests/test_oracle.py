@@ -115,6 +115,35 @@ def test_project_against_explicit_product():
         assert not np.isnan(S).any()  # every block assigned exactly once
 
 
+def test_panelwise_cholqr_against_the_row_sweeps():
+    # gram_panelwise is A^H A; rdiv_panelwise is Y R^-1 (a dropped update
+    # term or a transposed block fails the residual); cholqr2_panelwise meets
+    # the QR invariants and agrees with the row-sweep cholqr2.
+    rng = np.random.default_rng(21)
+    N, k = 200, 23
+    A = randmatrix(rng, N, k)
+    for w in (23, 12, 7, 1):
+        G = OB.gram_panelwise(A, w)
+        assert np.array_equal(np.tril(G, -1), np.triu(G, 1).conj().T)
+        assert OB.isapprox(G, A.conj().T @ A, 1e-14)
+        R, _ = OB.cholesky_upper(A.conj().T @ A, N, k)
+        X = OB.rdiv_panelwise(A, R, w)
+        assert np.linalg.norm(X @ R - A) <= 1e-13 * np.linalg.norm(A)
+        Q, Rq = OB.cholqr2_panelwise(A, w)
+        assert OB.orthogonality_error(Q) <= OB.orthogonality_bound(np.complex128, N, k)
+        assert np.linalg.norm(Q @ Rq - A) <= 1e-13 * np.linalg.norm(A)
+        assert np.allclose(Rq, np.triu(Rq)) and (np.diag(Rq).real > 0).all()
+        Q2, R2 = OB.cholqr2(A, w)
+        assert OB.isapprox(Rq, R2, 1e-12) and OB.isapprox(Q, Q2, 1e-12)
+    # The shifted third pass: a condition number past eps^-1/2.
+    U = scipy.linalg.qr(randmatrix(rng, N, k), mode="economic")[0]
+    V = scipy.linalg.qr(randmatrix(rng, k, k))[0]
+    B = U @ np.diag(10.0 ** np.linspace(0, -10, k)) @ V.conj().T
+    Q, Rq = OB.cholqr2_panelwise(B, 7)
+    assert OB.orthogonality_error(Q) <= 1e-8
+    assert np.linalg.norm(Q @ Rq - B) <= 1e-10 * np.linalg.norm(B)
+
+
 def test_scale_known_answers():
     # (1 + 2i)(3 - i) = 5 + 5i exactly; real scalars scale each part.
     assert OB.scale(np.array([1 + 2j]), 3 - 1j)[0] == 5 + 5j
