@@ -546,6 +546,45 @@ def streamed(args):
             "N": N, "budgets": "device 64 GiB, host 60 GiB page-locked, nbuffers 2"}
 
 
+def streamed_narrow(args):
+    """The narrowed host tier (host_eltype ComplexF32, src/panels.jl:361-398):
+    the full 216^3 x 512 basis is held in page-locked host memory as
+    ComplexF32 (41.3 GB, within the 60 GiB budget), widened to ComplexF64 on
+    the device as each row block arrives and narrowed on the way back, so the
+    FP64 cholqr2 moves half the link bytes. Reports the PCIe-inclusive rate
+    and ||Q^H Q - I||_F of the stored (ComplexF32) Q."""
+    import funicular_hip as F
+    backend = F.hip_backend()
+    n = args.grid
+    N, k, w = n ** 3, args.k, args.w
+    plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=60 * 1024 ** 3,
+                           host_eltype=np.complex64)
+    t = time.perf_counter()
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    log(f"streamed-narrow: {N}x{k} ComplexF32 host basis allocated in {time.perf_counter() - t:.1f} s")
+    d = 6.0 - 0.09 * (1 + 0.05j)
+    G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
+    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    t = time.perf_counter()
+    F.panelmul(Y, G, omega)
+    backend.sync_all()
+    t_panelmul = time.perf_counter() - t
+    c0 = F._lib.counters()
+    t = time.perf_counter()
+    F.cholqr2(Y)
+    backend.sync_all()
+    t_chol = time.perf_counter() - t
+    c1 = F._lib.counters()
+    up, down = c1["h2d_bytes"] - c0["h2d_bytes"], c1["d2h_bytes"] - c0["d2h_bytes"]
+    orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+    log(f"streamed-narrow: cholqr2 {t_chol:.2f} s, {N / t_chol:.0f} rows/s, up {up / 1e9:.1f} GB, "
+        f"down {down / 1e9:.1f} GB, ||Q^H Q - I||_F of the stored Q {orth:.2e}")
+    return {"cholqr2_s": t_chol, "rows_per_s_pcie_inclusive": N / t_chol, "h2d_GB": up / 1e9,
+            "d2h_GB": down / 1e9, "link_GBps_during_cholqr2": (up + down) / t_chol / 1e9,
+            "panelmul_s": t_panelmul, "stored_q_orthogonality": orth, "N": N,
+            "budgets": "device 64 GiB, host 60 GiB page-locked, host_eltype ComplexF32, nbuffers 2"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -562,6 +601,8 @@ def main():
     ap.add_argument("--streamed", action="store_true",
                     help="also run the host-resident (streamed over PCIe) cholqr2 and report overlap")
     ap.add_argument("--stream-grid", type=int, default=180, help="grid edge of the streamed run")
+    ap.add_argument("--streamed-narrow", action="store_true",
+                    help="also run the full-size streamed cholqr2 with ComplexF32 host storage")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profile mode: only the per-launch kernels at the row-block shape")
     args = ap.parse_args()
@@ -696,11 +737,16 @@ def main():
         import tempfile
         with tempfile.TemporaryDirectory(prefix="funicular-disk-") as d:
             record["disk_tier"] = disk_bench(backend, d)
-    if args.streamed:
+    if args.streamed or args.streamed_narrow:
         del Y, omega, plan
         import gc
         gc.collect()
+    if args.streamed:
         record["streamed"] = streamed(args)
+        gc.collect()
+    if args.streamed_narrow:
+        record["streamed_narrow"] = streamed_narrow(args)
+        gc.collect()
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         record["cpu_baseline"] = cpu_baseline(k, w, args.cpu_rows)
