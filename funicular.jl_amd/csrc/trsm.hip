@@ -407,6 +407,18 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
   }
 }
 
+// The solve proper, with the folded factor M already in the workspace.
+template <typename TS, typename T, bool CPLX, bool CT = false>
+static int launch_wide(TS *Y, int64_t ldy, int64_t m, int k, const T *M, hipStream_t stream) {
+  if (m == 0) return FHIP_OK;
+  const int64_t kp = trsm_kp(k);
+  const int64_t grid = cdiv64(m, TW_ROWS);
+  if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
+  hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y, ldy,
+                     m, k, M, kp);
+  return check_launch("trsm_wide_kernel");
+}
+
 template <typename TS, typename T, bool CPLX, bool CT = false>
 static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_t ldr, T *M, hipStream_t stream) {
   const int nbk = (k + TW_CB - 1) / TW_CB;
@@ -421,12 +433,7 @@ static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_
     st = check_launch("trsm_offdiag_kernel");
     if (st) return st;
   }
-  if (m == 0) return FHIP_OK;
-  const int64_t grid = cdiv64(m, TW_ROWS);
-  if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
-  hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y, ldy,
-                     m, k, (const T *)M, kp);
-  return check_launch("trsm_wide_kernel");
+  return launch_wide<TS, T, CPLX, CT>(Y, ldy, m, k, M, stream);
 }
 
 }  // namespace fhip
@@ -505,4 +512,42 @@ extern "C" int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const 
       return launch_trsm<float, float, false, true>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)work, s);
   }
   return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runc: unknown type %d", type);
+}
+
+extern "C" int fhip_trsm_apply(void *Y, int64_t ldy, int64_t m, int64_t k, int type, int compute, int adjoint,
+                               const void *work, size_t work_bytes, void *stream) {
+  if (!valid_type(type) || !valid_type(compute))
+    return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_apply: unknown type %d / %d", type, compute);
+  if (!(compute == type || (type == FHIP_C64 && compute == FHIP_C32 && !adjoint)))
+    return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_apply: no kernel for storage type %d with compute type %d", type,
+                compute);
+  if (m < 0 || k < 0) return fail(FHIP_E_ARGUMENT, "fhip_trsm_apply: negative dimension");
+  if (k == 0 || m == 0) return FHIP_OK;
+  if (k > (1 << 16)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_apply: too many columns");
+  if (!Y) return fail(FHIP_E_ARGUMENT, "fhip_trsm_apply: null matrix pointer");
+  if (ldy < m) return fail(FHIP_E_ARGUMENT, "fhip_trsm_apply: leading dimension smaller than the matrix");
+  size_t need = 0;
+  fhip_trsm_workspace_bytes(&need, k, compute);
+  if (!work || work_bytes < need)
+    return fail(FHIP_E_WORKSPACE, "fhip_trsm_apply: workspace of %zu bytes is smaller than the %zu needed",
+                work_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  const int ik = (int)k;
+  if (type == FHIP_C64 && compute == FHIP_C32)
+    return launch_wide<double, float, true>((double *)Y, ldy, m, ik, (const float *)work, s);
+  if (adjoint) {
+    switch (type) {
+      case FHIP_C64: return launch_wide<double, double, true, true>((double *)Y, ldy, m, ik, (const double *)work, s);
+      case FHIP_C32: return launch_wide<float, float, true, true>((float *)Y, ldy, m, ik, (const float *)work, s);
+      case FHIP_F64: return launch_wide<double, double, false, true>((double *)Y, ldy, m, ik, (const double *)work, s);
+      case FHIP_F32: return launch_wide<float, float, false, true>((float *)Y, ldy, m, ik, (const float *)work, s);
+    }
+  }
+  switch (type) {
+    case FHIP_C64: return launch_wide<double, double, true>((double *)Y, ldy, m, ik, (const double *)work, s);
+    case FHIP_C32: return launch_wide<float, float, true>((float *)Y, ldy, m, ik, (const float *)work, s);
+    case FHIP_F64: return launch_wide<double, double, false>((double *)Y, ldy, m, ik, (const double *)work, s);
+    case FHIP_F32: return launch_wide<float, float, false>((float *)Y, ldy, m, ik, (const float *)work, s);
+  }
+  return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_apply: unknown type %d", type);
 }

@@ -56,12 +56,16 @@ def rrule_cholqr2(Y, *, nbuffers=None):
             backend.h2d(Nd, np.asfortranarray(Nm.astype(T)))
             backend.h2d(Rd, np.asfortranarray(R.astype(T)))
 
-            def step(_, Qb, *rest):
+            def step(block, Qb, *rest):
                 Yb = rest[-1]
                 backend.rightmul_gemm(Yb, Qb, Nd)
                 if Qbar is not None:
                     backend.axpby(Yb, rest[0], 1.0, 1.0)
-                backend.rdiv_upper_adjoint(Yb, Rd, work)
+                # The first block folds R^H into the workspace, the rest reuse it.
+                if block == 0:
+                    backend.rdiv_upper_adjoint(Yb, Rd, work)
+                else:
+                    backend.rdiv_upper_folded(Yb, work, adjoint=True)
             operands = [readpanels(Q)] + ([readpanels(Qbar)] if Qbar is not None else []) + [writepanels(Ybar)]
             sweep(step, *operands, nbuffers=nbuffers, traversal=rowtraversal(Q))
         finally:

@@ -526,6 +526,21 @@ class HIPBackend(DeviceBackend):
              work_ptr, work_bytes, current_stream())
         return Y
 
+    def rdiv_upper_folded(self, Y, work, compute=None, adjoint=False):
+        """The solve of the last rdiv_upper (or rdiv_upper_adjoint) on another
+        block, reusing the folded factor it left in `work`."""
+        if Y.size == 0:
+            return Y
+        if work is None:
+            raise ArgumentError("rdiv_upper_folded needs the workspace a previous solve folded the factor into")
+        code = eltype_code(Y.dtype)
+        ccode = code if compute is None else eltype_code(compute)
+        need = self.trsm_workspace_bytes(Y.cols, Y.dtype if compute is None else compute)
+        work_ptr, work_bytes = _workspace(self, work, need)
+        call("fhip_trsm_apply", Y.ptr, Y.ld, Y.rows, Y.cols, code, ccode, 1 if adjoint else 0, work_ptr, work_bytes,
+             current_stream())
+        return Y
+
     def rdiv_upper_adjoint(self, Y, R, work=None):
         """Y <- Y R^-H for upper triangular R (the solve of the QR adjoint)."""
         if not (R.rows == R.cols == Y.cols):

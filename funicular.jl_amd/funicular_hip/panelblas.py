@@ -173,8 +173,14 @@ def rdiv_rows(Y, R, *, nbuffers=None, compute=None):
     try:
         work = _plan.checkout_workspace(plan, backend.trsm_workspace_bytes(Y.k, T if compute is None else compute))
         backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
-        sweep(lambda _, Yb: backend.rdiv_upper(Yb, factor, work, compute), updatepanels(Y), nbuffers=nbuffers,
-              traversal=rowtraversal(Y))
+        # The first block folds the factor into the workspace; the later ones,
+        # queued behind it on the compute stream, reuse it.
+        def step(block, Yb):
+            if block == 0:
+                backend.rdiv_upper(Yb, factor, work, compute)
+            else:
+                backend.rdiv_upper_folded(Yb, work, compute)
+        sweep(step, updatepanels(Y), nbuffers=nbuffers, traversal=rowtraversal(Y))
     finally:
         _plan.checkin_device_buffers(plan, [factor, work])
     return Y

@@ -172,8 +172,9 @@ int fhip_trsm_workspace_bytes(size_t *nbytes, int64_t k, int compute);
 
 /* Y <- Y * R^-1 in place, R upper triangular k x k with a non-unit diagonal
  * (right side, upper, no transpose). Y is m x k (ldy), R is k x k (ldr), both
- * of eltype `type`. Rows are independent. The solve is blocked by 32 columns
- * with the diagonal blocks inverted once per call. `compute` equals `type`,
+ * of eltype `type`. Rows are independent. The solve is blocked by 64 columns;
+ * each call first folds the inverted diagonal blocks into the factor, kept in
+ * the workspace (m = 0 does only that). `compute` equals `type`,
  * or is FHIP_C32 for a FHIP_C64 Y: the mixed first pass, which narrows Y and
  * R while staging, solves on the FP32 matrix cores and writes FP64 back.
  * Replaces rdiv_upper! (src/kernels.jl:108; semantics src/kernels.jl:90-99). */
@@ -188,6 +189,15 @@ int fhip_trsm_runn(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R,
  * is fhip_trsm_workspace_bytes(k, type). */
 int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr,
                    int type, void *work, size_t work_bytes, void *stream);
+
+/* The solve of fhip_trsm_runn (adjoint = 0) or fhip_trsm_runc (adjoint = 1)
+ * on another m x k block, reusing the folded factor that the last such call
+ * with the same k, type and compute left in `work` (on the same stream, or
+ * ordered before this one). A row sweep folds the factor on its first block
+ * only; the reference's rdiv_rows! solves every block with the same R
+ * (src/panelblas.jl:157-169). */
+int fhip_trsm_apply(void *Y, int64_t ldy, int64_t m, int64_t k, int type, int compute, int adjoint,
+                    const void *work, size_t work_bytes, void *stream);
 
 /* D <- A * C, A m x s (lda), C s x r (ldc), D m x r (ldd). D must not alias A.
  * Replaces rightmul_gemm! (src/kernels.jl:73; semantics src/kernels.jl:57-64). */

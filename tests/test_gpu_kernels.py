@@ -129,6 +129,36 @@ def test_rightmul_gemm_shapes(backend, m, s, r):
 
 
 @pytest.mark.parametrize("T", EL)
+@pytest.mark.parametrize("k", (5, 64, 130))
+def test_rdiv_upper_folded_reuses_the_factor(backend, T, k):
+    # A first solve folds R into the workspace; later blocks reuse it (R^H too).
+    rng = np.random.default_rng(k)
+    R = np.asfortranarray(np.triu(randmatrix(rng, k, k, T)) + 4 * np.eye(k, dtype=T))
+    Rd = backend.todevice(R)
+    work = backend.alloc_device(np.uint8, (backend.trsm_workspace_bytes(k, T), 1))
+    for adjoint in (False, True):
+        Rs = R.conj().T if adjoint else R
+        blocks = [randmatrix(rng, m, k, T) for m in (300, 77, 1)]
+        dev = [backend.todevice(Y) for Y in blocks]
+        if adjoint:
+            backend.rdiv_upper_adjoint(dev[0], Rd, work)
+        else:
+            backend.rdiv_upper(dev[0], Rd, work)
+        for d in dev[1:]:
+            backend.rdiv_upper_folded(d, work, adjoint=adjoint)
+        for Y, d in zip(blocks, dev):
+            X = backend.to_host(d)
+            assert np.linalg.norm(X @ Rs - Y) <= tol(T, Y.shape[0], k) * np.linalg.norm(Y)
+    if T == np.complex128:
+        Y = randmatrix(rng, 200, k, T)
+        d0, d1 = backend.todevice(Y), backend.todevice(Y)
+        w32 = backend.alloc_device(np.uint8, (backend.trsm_workspace_bytes(k, np.complex64), 1))
+        backend.rdiv_upper(d0, Rd, w32, compute=np.complex64)
+        backend.rdiv_upper_folded(d1, w32, compute=np.complex64)
+        assert np.array_equal(backend.to_host(d0), backend.to_host(d1))
+
+
+@pytest.mark.parametrize("T", EL)
 def test_rdiv_upper(backend, T):
     rng = np.random.default_rng(16)
     Y = randmatrix(rng, 7, 5, T)
