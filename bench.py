@@ -429,6 +429,64 @@ def disk_bench(backend, scratch):
     return out
 
 
+def range_finder_bench(backend, scratch, n=160, k=256, w=32, iterations=2):
+    """The disk-tier randomized range finder at a reduced size: G = the 3-D
+    7-point Helmholtz stencil on an n^3 grid, Omega = GhostPanels, Y = G Omega,
+    then `iterations` rounds of (Z = G^H Y, orthonormalize Z, Y = G Z,
+    orthonormalize Y), every orthonormalization cholqr2_panelwise with the
+    mixed first pass. Y and Z live on the scratch disk behind a host budget of
+    4 of their 8 panels, so the row sweeps of cholqr2 could not run at all.
+    Reports the time of each step, the disk traffic and ||Q^H Q - I||_F."""
+    import funicular_hip as F
+    N = n ** 3
+    panel = N * w * 16
+    plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=4 * panel + (1 << 20),
+                           scratch_dir=scratch, panel_width=w)
+    d = 6.0 - 0.09 * (1 + 0.05j)
+    G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
+    Gh = G.adjoint()
+    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    Z = F.PanelMatrix(N, k, plan=plan, w=w)
+    steps = []
+
+    def step(name, fn):
+        r0 = F.disk_reads(Y) + F.disk_reads(Z)
+        w0 = F.disk_writes(Y) + F.disk_writes(Z)
+        backend.sync_all()
+        t = time.perf_counter()
+        fn()
+        backend.sync_all()
+        dt = time.perf_counter() - t
+        reads = F.disk_reads(Y) + F.disk_reads(Z) - r0
+        writes = F.disk_writes(Y) + F.disk_writes(Z) - w0
+        steps.append({"step": name, "s": dt, "disk_read_GB": reads * panel / 1e9,
+                      "disk_write_GB": writes * panel / 1e9})
+        log(f"range finder: {name} {dt:.2f} s, disk {reads * panel / 1e9:.0f} GB read, "
+            f"{writes * panel / 1e9:.0f} GB written")
+
+    t0 = time.perf_counter()
+    step("Y = G Omega", lambda: F.panelmul(Y, G, omega))
+    step("cholqr2(Y)", lambda: F.cholqr2_panelwise(Y, mixed=True))
+    for it in range(iterations):
+        step(f"Z = G^H Y ({it + 1})", lambda: F.panelmul(Z, Gh, Y))
+        step(f"cholqr2(Z) ({it + 1})", lambda: F.cholqr2_panelwise(Z, mixed=True))
+        step(f"Y = G Z ({it + 1})", lambda: F.panelmul(Y, G, Z))
+        step(f"cholqr2(Y) ({it + 1})", lambda: F.cholqr2_panelwise(Y, mixed=True))
+    total = time.perf_counter() - t0
+    orth = float(np.linalg.norm(F.gram_panelwise(Y) - np.eye(k)))
+    read = sum(s["disk_read_GB"] for s in steps)
+    written = sum(s["disk_write_GB"] for s in steps)
+    out = {"workload": f"range finder on a {n}^3 Helmholtz grid: N={N}, k={k}, panels of {w} ({panel / 1e9:.2f} GB), "
+                       f"{iterations} power iterations, mixed first pass, host budget 4 of 8 panels",
+           "total_s": total, "steps": steps, "disk_read_GB": read, "disk_write_GB": written,
+           "disk_GBps": (read + written) / total, "orthogonality": orth}
+    log(f"range finder: {total:.1f} s, ||Q^H Q - I||_F = {orth:.2e}")
+    for m in (Y, Z):
+        F.free(m)
+    return out
+
+
 def kernels_only(rows, k, reps):
     """The per-launch shapes of the benchmark, without the pipeline around
     them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
@@ -606,6 +664,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload (N = 2^22, k = 128)")
     ap.add_argument("--disk", action="store_true", help="also measure the disk tier (17 GB scratch file)")
+    ap.add_argument("--range-finder", action="store_true",
+                    help="also run the disk-tier range finder with power iterations (34 GB of scratch files)")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
     ap.add_argument("--streamed", action="store_true",
                     help="also run the host-resident (streamed over PCIe) cholqr2 and report overlap")
@@ -747,6 +807,10 @@ def main():
         import tempfile
         with tempfile.TemporaryDirectory(prefix="funicular-disk-") as d:
             record["disk_tier"] = disk_bench(backend, d)
+    if args.range_finder:
+        import tempfile
+        with tempfile.TemporaryDirectory(prefix="funicular-rf-") as d:
+            record["range_finder"] = range_finder_bench(backend, d)
     if args.streamed or args.streamed_narrow:
         del Y, omega, plan
         import gc

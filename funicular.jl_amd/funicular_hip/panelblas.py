@@ -317,10 +317,11 @@ def panel_schedule_gram(p):
     return panels, labels
 
 
-def gram_panelwise(X, *, nbuffers=None):
+def gram_panelwise(X, *, nbuffers=None, compute=None):
     """The Hermitian k x k Gram X^H X from pairs of column panels: block (i, j)
     of the upper triangle is X_i^H X_j, formed with panel j held on the device.
-    The lower triangle is mirrored on the host."""
+    The lower triangle is mirrored on the host. compute=np.complex64 forms the
+    blocks on the FP32 matrix cores, as gram does."""
     plan = X.plan
     backend = plan.backend
     T = X.dtype
@@ -330,8 +331,8 @@ def gram_panelwise(X, *, nbuffers=None):
     acc = work = None
     try:
         acc = _plan.checkout_device_buffers(plan, T, (X.k, X.k), 1)[0]
-        need = max(backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=True),
-                   backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=False))
+        need = max(backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=True, compute=compute),
+                   backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=False, compute=compute))
         work = _plan.checkout_workspace(plan, need)
 
         def step(label, Xb):
@@ -340,10 +341,12 @@ def gram_panelwise(X, *, nbuffers=None):
             held = hold.view(None, (0, panelwidth(X, j)))
             if first:
                 backend.convert_copy(held, Xb)
-                backend.gram_accumulate(acc.view((rj.start, rj.stop), (rj.start, rj.stop)), Xb, Xb, 1.0, 0.0, work)
+                backend.gram_accumulate(acc.view((rj.start, rj.stop), (rj.start, rj.stop)), Xb, Xb, 1.0, 0.0, work,
+                                        compute)
             else:
                 ri = panelrange(X, i)
-                backend.gram_accumulate(acc.view((ri.start, ri.stop), (rj.start, rj.stop)), Xb, held, 1.0, 0.0, work)
+                backend.gram_accumulate(acc.view((ri.start, ri.stop), (rj.start, rj.stop)), Xb, held, 1.0, 0.0, work,
+                                        compute)
         sweep(step, readpanels(X), nbuffers=nbuffers, traversal=PanelSteps(panels, labels))
         G = collect_small(acc, plan)
     finally:
@@ -390,17 +393,29 @@ def rdiv_panelwise(Y, R, *, nbuffers=None):
     return Y
 
 
-def cholqr2_panelwise(Y, *, nbuffers=None):
+def cholqr2_panelwise(Y, *, nbuffers=None, mixed=False, shift="adaptive"):
     """cholqr2 by column panels (gram_panelwise, host Cholesky,
     rdiv_panelwise), for a basis whose panels do not all fit the host budget.
     Two passes and a third when a Cholesky needed a shift, as cholqr2.
-    Returns R with Q R equal to the original Y; Y holds Q."""
+    mixed=True makes the first pass the mixed one of cholqr2_mixed with
+    solve="fp64": its Gram on the FP32 matrix cores and its Cholesky in FP32
+    (shifted as `shift` says), the solve in FP64. Returns R with Q R equal to
+    the original Y; Y holds Q."""
     def one_pass():
         G = gram_panelwise(Y, nbuffers=nbuffers)
         R, shifted = cholesky_upper(G, Y.N, Y.k)
         rdiv_panelwise(Y, R, nbuffers=nbuffers)
         return R, shifted
-    R, shifted = one_pass()
+    if mixed:
+        if Y.dtype != np.dtype(np.complex128):
+            raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
+        G = gram_panelwise(Y, nbuffers=nbuffers, compute=np.complex64)
+        R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
+        R = R32.astype(Y.dtype)
+        rdiv_panelwise(Y, R, nbuffers=nbuffers)
+        shifted = s > 0.0
+    else:
+        R, shifted = one_pass()
     R2, shifted2 = one_pass()
     R = triangular_product(R2, R)
     if shifted or shifted2:

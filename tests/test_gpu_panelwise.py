@@ -126,3 +126,26 @@ def test_panelwise_matches_the_row_sweeps_on_a_ghost_basis(backend):
         assert np.linalg.norm(G - np.eye(k)) <= 1e-13
     for Y in (Y1, Y2):
         F.free(Y)
+
+
+def test_panelwise_mixed_first_pass(backend, tmp_path):
+    # The FP32 Gram and Cholesky only precondition: after the FP64 pass Q is
+    # orthonormal to FP64 level and Q R reproduces Y to FP64 rounding (the
+    # solve stays FP64), on a disk-backed basis past the host budget.
+    rng = np.random.default_rng(13)
+    N, k, w = 6000, 48, 8
+    U = scipy.linalg.qr(randmatrix(rng, N, k), mode="economic")[0]
+    V = scipy.linalg.qr(randmatrix(rng, k, k))[0]
+    A = np.asfortranarray(U @ np.diag(10.0 ** np.linspace(0, -2, k)) @ V.conj().T)
+    budget = 4 * P.align_up(P.panel_bytes(N, w, np.dtype(C16)), P.SLAB_ALIGN)
+    plan = F.ResidencyPlan(backend=backend, device_budget=64 << 20, host_budget=budget,
+                           scratch_dir=str(tmp_path), panel_width=w)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    R = F.cholqr2_panelwise(Y, mixed=True)
+    Q = F.to_matrix(Y)
+    assert OB.orthogonality_error(Q) <= OB.orthogonality_bound(C16, N, k)
+    assert np.linalg.norm(Q @ R - A) <= OB.blastol(C16, N, k) * np.linalg.norm(A)
+    Qo, Ro = OB.cholqr2(A, w)
+    assert OB.isapprox(R, Ro, 1e-12)
+    with pytest.raises(F.ArgumentError):
+        F.cholqr2_panelwise(F.PanelMatrix(100, 4, plan=plan, w=2, dtype=np.complex64), mixed=True)
