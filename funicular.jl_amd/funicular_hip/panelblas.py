@@ -15,7 +15,8 @@ from . import plan as _plan
 from .device_api import ArgumentError
 from .executor import PanelSteps, RowSteps, readpanels, rowtraversal, sweep, updatepanels, writepanels
 from .operator import apply_operator, assert_operator_shape, assert_square_operator
-from .panels import (assert_conformal, assert_same_panels, assert_same_rows, npanels, panelrange, panelwidth)
+from .panels import (assert_conformal, assert_same_panels, assert_same_rows, npanels, panelrange, panelwidth,
+                     pooled_like)
 
 
 def shared_plan(X, Y):
@@ -267,10 +268,27 @@ def project_schedule(p):
     return panels, labels
 
 
+def _pool_can_hold(plan, T, dims):
+    pool = plan.devicepool
+    if pool.available.get((np.dtype(T), (int(dims[0]), int(dims[1])))):
+        return True
+    return pool.allocated + int(dims[0]) * int(dims[1]) * np.dtype(T).itemsize <= pool.budget
+
+
 def project(Q, G, *, nbuffers=None):
-    """S = Q^H G Q without storing G Q (src/panelblas.jl:247-301)."""
+    """S = Q^H G Q without storing G Q (src/panelblas.jl:247-301).
+
+    A basis resident in HBM whose plan can lend a second N x k buffer takes a
+    shorter route, since no panel has to travel: G Q is formed whole in that
+    pooled buffer, one stencil pass per panel, and a single Gram sweep gives
+    all of S on 64 x 64 matrix-core tiles, instead of p^2 reductions of 32 x 32
+    tiles, each reading two whole panels from HBM."""
     plan = Q.plan
     assert_square_operator(G, Q.N)
+    if Q.resident and _pool_can_hold(plan, Q.dtype, (Q.N, Q.k)):
+        with pooled_like(Q) as GQ:
+            panelmul(GQ, G, Q, nbuffers=nbuffers)
+            return gram(Q, GQ, nbuffers=nbuffers)
     p = npanels(Q)
     T = Q.dtype
     backend = plan.backend

@@ -233,6 +233,31 @@ def test_device_resident_plan(backend, nb):
     assert np.array_equal(Q, F.to_matrix(Yh))
 
 
+@pytest.mark.parametrize("w", (23, 8, 1))
+def test_project_on_a_device_resident_plan(backend, w):
+    # With room for a second N x k buffer, G Q is formed whole and S is one
+    # Gram sweep; with no such room, the reference's panel schedule runs.
+    print(f"resident project w={w}", flush=True)
+    rng = np.random.default_rng(700 + w)
+    N, k = 20000, 64
+    A = randmatrix(rng, N, k)
+    tol = OB.blastol(C16, N, k)
+    d, o = 2 + 0.1j, -1.0
+    ref = OB.project(A, lambda X: OO.tridiag_apply(X, d, o), w)
+    matrix = N * k * 16
+    # The plan needs two N x w buffers besides the matrix; w < k / 2 leaves
+    # the second budget short of a second N x k.
+    for budget, pooled in ((4 * matrix, True), (matrix + max(2 * N * w * 16, 10 << 20), False)):
+        plan = make_plan(backend, device_budget=budget, device_resident=True)
+        G = F.Tridiagonal1D(N, d, o, backend=backend)
+        Q = F.PanelMatrix.from_array(A, plan=plan, w=w)
+        S = F.project(Q, G)
+        assert OB.isapprox(S, ref, tol)
+        assert np.array_equal(F.to_matrix(Q), A)
+        # The pooled route leaves its N x k buffer in the pool for reuse.
+        assert bool(plan.devicepool.available.get((np.dtype(C16), (N, k)))) == pooled
+
+
 @pytest.mark.parametrize("w", (23, 8, 7, 1))
 @pytest.mark.parametrize("nb", (1, 2))
 def test_sweeps_move_every_panel(backend, w, nb):
