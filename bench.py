@@ -487,7 +487,45 @@ def range_finder_bench(backend, scratch, n=160, k=256, w=32, iterations=2):
     return out
 
 
-def kernels_only(rows, k, reps):
+def vendor_timings(backend, A, B, C, R, rows, k, timed):
+    """rocBLAS on the same shapes, for comparison: zherk and a plain zgemm
+    (A^H A, as the reference's CUDA path forms its Gram) for the Gram, ztrsm
+    (right, upper, no transpose) for the solve, zgemm for the row-block
+    product. Same HIP-event timing as the kernels above; TFLOP/s of zgemm_gram
+    count the full 8 N k^2 flops it performs."""
+    import ctypes
+    lib = ctypes.CDLL("librocblas.so")
+    handle = ctypes.c_void_p()
+    if lib.rocblas_create_handle(ctypes.byref(handle)) != 0:
+        raise RuntimeError("rocblas_create_handle failed")
+    lib.rocblas_set_stream(handle, None)
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    one_r, zero_r = ctypes.c_double(1.0), ctypes.c_double(0.0)
+    one_c, zero_c = (ctypes.c_double * 2)(1.0, 0.0), (ctypes.c_double * 2)(0.0, 0.0)
+    FILL_UPPER, OP_N, OP_C, SIDE_R, DIAG_N = 121, 111, 113, 142, 131
+
+    def check(st):
+        if st != 0:
+            raise RuntimeError(f"rocBLAS status {st}")
+    lib.rocblas_zherk.argtypes = [vp, i32, i32, i32, i32, vp, vp, i32, vp, vp, i32]
+    lib.rocblas_ztrsm.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32]
+    lib.rocblas_zgemm.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp, i32]
+    timed("rocblas_zherk", lambda: check(lib.rocblas_zherk(handle, FILL_UPPER, OP_C, k, rows, ctypes.byref(one_r),
+                                                           A.ptr, A.ld, ctypes.byref(zero_r), C.ptr, C.ld)),
+          gram_flops(rows, k))
+    # The reference's CUDA path forms the Gram with a plain ZGEMM (A^H B).
+    timed("rocblas_zgemm_gram", lambda: check(lib.rocblas_zgemm(handle, OP_C, OP_N, k, k, rows, one_c, A.ptr, A.ld,
+                                                                A.ptr, A.ld, zero_c, C.ptr, C.ld)),
+          8.0 * rows * k * k)
+    timed("rocblas_ztrsm", lambda: check(lib.rocblas_ztrsm(handle, SIDE_R, FILL_UPPER, OP_N, DIAG_N, rows, k, one_c,
+                                                           R.ptr, R.ld, B.ptr, B.ld)), trsm_flops(rows, k))
+    timed("rocblas_zgemm", lambda: check(lib.rocblas_zgemm(handle, OP_N, OP_N, rows, k, k, one_c, A.ptr, A.ld, R.ptr,
+                                                           R.ld, zero_c, B.ptr, B.ld)), 8.0 * rows * k * k)
+    backend.sync_all()
+    lib.rocblas_destroy_handle(handle)
+
+
+def kernels_only(rows, k, reps, vendor=False):
     """The per-launch shapes of the benchmark, without the pipeline around
     them: one plain copy of a rows x k ComplexF64 block (the calibration kernel
     for the HBM counters), then `reps` FP64 Grams, FP32 (mixed) Grams, FP64 and
@@ -546,6 +584,8 @@ def kernels_only(rows, k, reps):
     timed("scale_fp64", lambda: backend.scale(B, 0.6 + 0.8j), 0, 2 * blk)
     timed("axpby_fp64", lambda: backend.axpby(B, A, 0.5, 1.0), 0, 3 * blk)
     backend.sync_all()
+    if vendor:
+        vendor_timings(backend, A, B, C, R, rows, k, timed)
     log(f"kernels-only: {rows}x{k}, {reps} repetitions of each kernel")
     print(json.dumps(out), flush=True)
 
@@ -674,9 +714,11 @@ def main():
                     help="also run the full-size streamed cholqr2 with ComplexF32 host storage")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profile mode: only the per-launch kernels at the row-block shape")
+    ap.add_argument("--vendor", action="store_true",
+                    help="with --kernels-only: also time rocBLAS zherk / ztrsm / zgemm on the same shapes")
     args = ap.parse_args()
     if args.kernels_only:
-        kernels_only(args.grid ** 3 * args.w // args.k, args.k, max(args.steps, 1))
+        kernels_only(args.grid ** 3 * args.w // args.k, args.k, max(args.steps, 1), vendor=args.vendor)
         return
 
     if args.gpus != 1:
