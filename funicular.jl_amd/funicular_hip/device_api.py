@@ -526,6 +526,20 @@ class HIPBackend(DeviceBackend):
              work_ptr, work_bytes, current_stream())
         return Y
 
+    def potrf_upper(self, R, G, info):
+        """R <- the upper Cholesky factor of Hermitian G (upper triangle read),
+        on the current stream; `info` is a 1 x 1 int32 device matrix that
+        becomes 0 or the 1-based column of the first bad pivot. Returns
+        without waiting; read `info` after the stream."""
+        k = G.rows
+        if not (G.cols == k and R.shape == (k, k)):
+            raise DimensionMismatch(f"cannot factor a {G.shape} matrix into a {R.shape} one")
+        if info.dtype != np.int32 or info.size < 1:
+            raise ArgumentError("potrf_upper needs an int32 device scalar for info")
+        call("fhip_potrf_upper", R.ptr, R.ld, eltype_code(R.dtype), G.ptr, G.ld, eltype_code(G.dtype), k, info.ptr,
+             current_stream())
+        return R
+
     def rdiv_upper_folded(self, Y, work, compute=None, adjoint=False):
         """The solve of the last rdiv_upper (or rdiv_upper_adjoint) on another
         block, reusing the folded factor it left in `work`."""

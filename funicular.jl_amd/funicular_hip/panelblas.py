@@ -46,19 +46,28 @@ def gram(X, Y=None, *, nbuffers=None, compute=None):
     """The k x k matrix X^H Y, accumulated on the device one row block at a
     time and copied back once (src/panelblas.jl:10-53). `compute=np.complex64`
     forms it on the FP32 matrix cores with FP64 sums across row chunks."""
-    same = Y is None or Y is X
     Y = X if Y is None else Y
     plan = shared_plan(X, Y)
     assert_conformal(X, Y)
+    accumulator = _plan.checkout_device_buffers(plan, X.dtype, (X.k, X.k), 1)[0]
+    try:
+        gram_into(accumulator, X, Y, nbuffers=nbuffers, compute=compute)
+        return collect_small(accumulator, plan)
+    finally:
+        _plan.checkin_device_buffers(plan, [accumulator])
+
+
+def gram_into(accumulator, X, Y, *, nbuffers=None, compute=None):
+    """gram's row sweep into a k x k device accumulator, left on the device."""
+    same = Y is X
+    plan = X.plan
     T = X.dtype
     backend = plan.backend
     traversal = rowtraversal(X)
     height = traversal.blocks[0][1] - traversal.blocks[0][0]
-    accumulator = _plan.checkout_device_buffers(plan, T, (X.k, X.k), 1)[0]
-    work = None
+    work = _plan.checkout_workspace(
+        plan, backend.gram_workspace_bytes(height, X.k, X.k, T, hermitian=same, compute=compute))
     try:
-        work = _plan.checkout_workspace(
-            plan, backend.gram_workspace_bytes(height, X.k, X.k, T, hermitian=same, compute=compute))
         if same:
             def step(block, Xb):
                 backend.gram_accumulate(accumulator, Xb, Xb, 1.0, carry(block), work, compute)
@@ -67,9 +76,9 @@ def gram(X, Y=None, *, nbuffers=None, compute=None):
             def step(block, Xb, Yb):
                 backend.gram_accumulate(accumulator, Xb, Yb, 1.0, carry(block), work, compute)
             sweep(step, readpanels(X), readpanels(Y), nbuffers=nbuffers, traversal=traversal)
-        return collect_small(accumulator, plan)
     finally:
-        _plan.checkin_device_buffers(plan, [accumulator, work])
+        _plan.checkin_device_buffers(plan, [work])
+    return accumulator
 
 
 def panelmul(Y, G, X, *, nbuffers=None):
@@ -108,30 +117,77 @@ def triangular_product(A, B):
 _product_pool = None
 
 
+def _device_pass(Y, nbuffers, R_prev=None, first=None):
+    """One CholQR pass with G, R and the solve's factor kept on the device
+    (src/panelblas.jl:141-146): the Gram sweep into a device accumulator, its
+    plain Cholesky factorization on the device, the solve sweep from the
+    device factor. Only R comes back to the host (4 MiB at k = 512), where the
+    product R R_prev is formed on a thread while the solve runs. When the
+    device factorization meets a bad pivot, G goes to the host and the
+    reference's rule takes over there (a plain retry, then the shift).
+
+    first=(shift, solve) makes it the mixed first pass of cholqr2_mixed: FP32
+    Gram and Cholesky, the solve in FP32 or FP64. Returns (R_total, shifted)
+    for an FP64 pass and (R, s) for the mixed one."""
+    global _product_pool
+    plan = Y.plan
+    backend = plan.backend
+    T = Y.dtype
+    k = Y.k
+    acc, factor = _plan.checkout_device_buffers(plan, T, (k, k), 2)
+    info = narrow = None
+    try:
+        info = _plan.checkout_device_buffers(plan, np.int32, (1, 1), 1)[0]
+        compute = None if first is None else np.complex64
+        gram_into(acc, Y, Y, nbuffers=nbuffers, compute=compute)
+        shifted, s = False, 0.0
+        bad = 1
+        if first is None:
+            backend.potrf_upper(factor, acc, info)
+            bad = int(backend.to_host(info)[0, 0])
+        elif first[0] == "adaptive":
+            narrow = _plan.checkout_device_buffers(plan, np.complex64, (k, k), 1)[0]
+            backend.potrf_upper(narrow, acc, info)
+            bad = int(backend.to_host(info)[0, 0])
+            if bad == 0:
+                backend.convert_copy(factor, narrow)
+        if bad == 0:
+            R = backend.to_host(factor)
+        else:
+            G = collect_small(acc, plan)
+            if first is None:
+                R, shifted = cholesky_upper(G, Y.N, k)
+            else:
+                R32, s = cholesky_upper_fp32(G, Y.N, k, first[0])
+                R = R32.astype(T)
+            backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
+        product = None
+        if R_prev is not None:
+            if _product_pool is None:
+                import concurrent.futures
+                _product_pool = concurrent.futures.ThreadPoolExecutor(max_workers=1,
+                                                                      thread_name_prefix="cholqr-product")
+            product = _product_pool.submit(triangular_product, R, R_prev)
+        solve = None if first is None or first[1] == "fp64" else np.complex64
+        try:
+            rdiv_rows(Y, R, nbuffers=nbuffers, compute=solve, factor=factor)
+        finally:
+            total = R if product is None else product.result()
+        return (total, shifted) if first is None else (R, s)
+    finally:
+        _plan.checkin_device_buffers(plan, [acc, factor, info, narrow])
+
+
 def cholqr_pass_then(Y, R_prev, nbuffers=None):
     """One more CholQR pass over Y, returning (R_pass R_prev, shifted). The
     host forms the triangular product while the pass's solve sweep runs on the
     device, instead of after it."""
-    global _product_pool
-    G = gram(Y, nbuffers=nbuffers)
-    R, shifted = cholesky_upper(G, Y.N, Y.k)
-    if _product_pool is None:
-        import concurrent.futures
-        _product_pool = concurrent.futures.ThreadPoolExecutor(max_workers=1, thread_name_prefix="cholqr-product")
-    product = _product_pool.submit(triangular_product, R, R_prev)
-    try:
-        rdiv_rows(Y, R, nbuffers=nbuffers)
-    finally:
-        total = product.result()
-    return total, shifted
+    return _device_pass(Y, nbuffers, R_prev=R_prev)
 
 
 def cholqr_pass(Y, nbuffers=None):
     """src/panelblas.jl:141-146."""
-    G = gram(Y, nbuffers=nbuffers)
-    R, shifted = cholesky_upper(G, Y.N, Y.k)
-    rdiv_rows(Y, R, nbuffers=nbuffers)
-    return R, shifted
+    return _device_pass(Y, nbuffers)
 
 
 def _potrf(G):
@@ -164,16 +220,20 @@ def cholesky_upper(G, N, k):
     return shifted, True
 
 
-def rdiv_rows(Y, R, *, nbuffers=None, compute=None):
-    """Y <- Y R^-1 over row blocks (src/panelblas.jl:157-169)."""
+def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None):
+    """Y <- Y R^-1 over row blocks (src/panelblas.jl:157-169). `factor`, a
+    device copy of R already in place, saves the upload."""
     plan = Y.plan
     T = Y.dtype
     backend = plan.backend
-    factor = _plan.checkout_device_buffers(plan, T, R.shape, 1)[0]
+    owned = factor is None
+    if owned:
+        factor = _plan.checkout_device_buffers(plan, T, R.shape, 1)[0]
     work = None
     try:
         work = _plan.checkout_workspace(plan, backend.trsm_workspace_bytes(Y.k, T if compute is None else compute))
-        backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
+        if owned:
+            backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
         # The first block folds the factor into the workspace; the later ones,
         # queued behind it on the compute stream, reuse it.
         def step(block, Yb):
@@ -187,7 +247,7 @@ def rdiv_rows(Y, R, *, nbuffers=None, compute=None):
         traversal = RowSteps(Y.N, Y.N) if Y.resident else rowtraversal(Y)
         sweep(step, updatepanels(Y), nbuffers=nbuffers, traversal=traversal)
     finally:
-        _plan.checkin_device_buffers(plan, [factor, work])
+        _plan.checkin_device_buffers(plan, [factor, work] if owned else [work])
     return Y
 
 
@@ -554,11 +614,9 @@ def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp32"):
         raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
     if solve not in ("fp64", "fp32"):
         raise ArgumentError(f"solve must be 'fp64' or 'fp32', got {solve!r}")
-    G = gram(Y, nbuffers=nbuffers, compute=np.complex64)
-    R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
-    R = R32.astype(Y.dtype)
-    rdiv_rows(Y, R, nbuffers=nbuffers, compute=np.complex64 if solve == "fp32" else None)
-    return R, s
+    if shift not in ("adaptive", "fukaya"):
+        raise ArgumentError(f"shift must be 'adaptive' or 'fukaya', got {shift!r}")
+    return _device_pass(Y, nbuffers, first=(shift, solve))
 
 
 def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32"):

@@ -189,7 +189,7 @@ def checkout_device_buffers(plan, T, dims, count):
     """src/plan.jl:159-175: `count` buffers of eltype T and shape dims, from the
     pool when it has them, otherwise allocated within the budget."""
     T = np.dtype(T)
-    if not plan.backend.supports_eltype(T) and T != np.dtype(np.uint8):
+    if not plan.backend.supports_eltype(T) and T not in (np.dtype(np.uint8), np.dtype(np.int32)):
         raise ArgumentError(f"backend {type(plan.backend).__name__} cannot compute in {T}")
     pool = plan.devicepool
     dims = (int(dims[0]), int(dims[1]))

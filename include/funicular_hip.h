@@ -199,6 +199,17 @@ int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, in
 int fhip_trsm_apply(void *Y, int64_t ldy, int64_t m, int64_t k, int type, int compute, int adjoint,
                     const void *work, size_t work_bytes, void *stream);
 
+/* R <- the upper Cholesky factor of the Hermitian (symmetric) k x k matrix G,
+ * whose upper triangle alone is read; R's lower triangle is set to zero. R's
+ * eltype equals G's, or narrows FHIP_C64 to FHIP_C32 (FHIP_F64 to FHIP_F32)
+ * for the mixed first pass. *info (a device int) becomes 0, or the 1-based
+ * column of the first pivot that is not positive and finite, as LAPACK's
+ * potrf reports it; the caller reads it after the stream. Replaces the plain
+ * factorization of cholesky_upper (src/panelblas.jl:148-155), which the
+ * reference runs on the host; its shifted retry stays on the host. */
+int fhip_potrf_upper(void *R, int64_t ldr, int rtype, const void *G, int64_t ldg, int gtype,
+                     int64_t k, int *info, void *stream);
+
 /* D <- A * C, A m x s (lda), C s x r (ldc), D m x r (ldd). D must not alias A.
  * Replaces rightmul_gemm! (src/kernels.jl:73; semantics src/kernels.jl:57-64). */
 int fhip_gemm_nn(void *D, int64_t ldd, const void *A, int64_t lda, const void *C,

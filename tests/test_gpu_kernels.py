@@ -1,6 +1,7 @@
 """The HIP kernels against the oracle, through the C ABI (src/kernels.jl
 semantics; test/test_kernels.jl is the model)."""
 import numpy as np
+import scipy.linalg
 import pytest
 
 from oracle import ghost as OG
@@ -156,6 +157,64 @@ def test_rdiv_upper_folded_reuses_the_factor(backend, T, k):
         backend.rdiv_upper(d0, Rd, w32, compute=np.complex64)
         backend.rdiv_upper_folded(d1, w32, compute=np.complex64)
         assert np.array_equal(backend.to_host(d0), backend.to_host(d1))
+
+
+def hpd(rng, k, T, cond=1e3):
+    Qm = np.linalg.qr(randmatrix(rng, k, k, T))[0]
+    G = (Qm * np.logspace(0, -np.log10(cond), k)) @ Qm.conj().T
+    return np.asfortranarray(((G + G.conj().T) / 2).astype(T))
+
+
+@pytest.mark.parametrize("T", EL)
+@pytest.mark.parametrize("k", (1, 5, 31, 32, 33, 64, 100, 512))
+def test_potrf_upper_against_lapack(backend, T, k):
+    # The lower triangle of G holds NaN: only the upper one may be read.
+    rng = np.random.default_rng(k)
+    G = hpd(rng, k, T)
+    Gn = G.copy()
+    Gn[np.tril_indices(k, -1)] = np.nan
+    Gd = backend.todevice(Gn)
+    Rd = backend.alloc_device(T, (k, k))
+    backend.fill(Rd, np.nan)
+    info = backend.alloc_device(np.int32, (1, 1))
+    backend.potrf_upper(Rd, Gd, info)
+    assert int(backend.to_host(info)[0, 0]) == 0
+    R = backend.to_host(Rd)
+    ref = scipy.linalg.cholesky(G.astype(np.complex128 if np.iscomplexobj(G) else np.float64), lower=False)
+    assert np.all(np.tril(R, -1) == 0)
+    assert np.all(np.diag(R).real > 0)
+    eps = np.finfo(T).eps
+    assert np.linalg.norm(R - ref) <= 50 * eps * k * np.linalg.norm(ref) * 1e3 ** 0.5
+    assert np.linalg.norm(R.conj().T @ R - G) <= 50 * eps * k * np.linalg.norm(G)
+
+
+def test_potrf_upper_narrows_and_reports_the_first_bad_pivot(backend):
+    rng = np.random.default_rng(3)
+    k = 100
+    G = hpd(rng, k, np.complex128)
+    # FP64 Gram factored in FP32, as the mixed first pass does.
+    Gd = backend.todevice(G)
+    R32 = backend.alloc_device(np.complex64, (k, k))
+    info = backend.alloc_device(np.int32, (1, 1))
+    backend.potrf_upper(R32, Gd, info)
+    assert int(backend.to_host(info)[0, 0]) == 0
+    R = backend.to_host(R32).astype(np.complex128)
+    assert np.linalg.norm(R.conj().T @ R - G) <= 1e-5 * np.linalg.norm(G)
+    # An indefinite leading minor of order 38 (1-based): LAPACK's info.
+    B = G.copy()
+    B[37, 37] = -1.0
+    potrf, = scipy.linalg.get_lapack_funcs(("potrf",), (B,))
+    _, lapack_info = potrf(B, lower=False)
+    Rd = backend.alloc_device(np.complex128, (k, k))
+    backend.potrf_upper(Rd, backend.todevice(B), info)
+    assert int(backend.to_host(info)[0, 0]) == lapack_info == 38
+    # A NaN on the diagonal stops it there too; a clean matrix resets info.
+    Cn = G.copy()
+    Cn[70, 70] = np.nan
+    backend.potrf_upper(Rd, backend.todevice(Cn), info)
+    assert int(backend.to_host(info)[0, 0]) == 71
+    backend.potrf_upper(Rd, Gd, info)
+    assert int(backend.to_host(info)[0, 0]) == 0
 
 
 @pytest.mark.parametrize("T", EL)
