@@ -1,0 +1,80 @@
+"""The 216^3 x 512 Helmholtz basis (N = 10,077,696, the benchmark's size),
+resident in HBM, through size-independent properties: the QR invariants
+Q^H Q = I and Q R = Y (Y regenerated panel by panel from its ghost seed), the
+mixed first pass's orthogonality bar, and S - S^H = 2 i Im(d) Q^H Q for the
+projected Helmholtz operator G = d I + (real symmetric stencil)."""
+import gc
+
+import numpy as np
+import pytest
+
+import funicular_hip as F
+
+pytestmark = pytest.mark.gpu
+
+N_EDGE, K, W = 216, 512, 32
+SEED = 0xDEADBEEF
+D = 6.0 - 0.09 * (1 + 0.05j)
+
+
+@pytest.fixture(scope="module")
+def basis(backend):
+    N = N_EDGE ** 3
+    plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=8 * 1024 ** 3,
+                           device_resident=True)
+    G = F.Helmholtz3D(N_EDGE, N_EDGE, N_EDGE, D, -1.0, backend=backend)
+    omega = F.GhostPanels(np.complex128, N, K, plan=plan, seed=SEED, w=W)
+    Y = F.PanelMatrix(N, K, plan=plan, w=W)
+    yield plan, G, omega, Y
+    F.free(Y)
+    del plan, G, omega, Y
+    gc.collect()
+
+
+def residual(backend, Y, R, G):
+    """||Y0 - Q R||_F / ||Y0||_F with Y0 = G Omega regenerated a panel at a time."""
+    N, k, w = Y.N, Y.k, Y.w
+    Rd = backend.todevice(np.asfortranarray(R))
+    y0, qr, om = (backend.alloc_device(np.complex128, (N, w)) for _ in range(3))
+    g = backend.alloc_device(np.complex128, (w, w))
+    num = den = 0.0
+    for j in range(0, k, w):
+        backend.ghost_fill(om, 0, j + 1, SEED)
+        G.apply(y0, om)
+        backend.rightmul_gemm(qr, Y.device.view(None, (0, j + w)), Rd.view((0, j + w), (j, j + w)))
+        backend.gram_accumulate(g, y0, y0, 1.0, 0.0)
+        den += float(np.trace(backend.to_host(g)).real)
+        backend.axpby(qr, y0, -1.0, 1.0)
+        backend.gram_accumulate(g, qr, qr, 1.0, 0.0)
+        num += float(np.trace(backend.to_host(g)).real)
+    return float(np.sqrt(num / den))
+
+
+def test_cholqr2_at_full_size(backend, basis):
+    plan, G, omega, Y = basis
+    print("full size: fp64 cholqr2", flush=True)
+    F.panelmul(Y, G, omega)
+    R = F.cholqr2(Y)
+    assert np.allclose(R, np.triu(R)) and (np.diag(R).real > 0).all()
+    orth = float(np.linalg.norm(F.gram(Y) - np.eye(K)))
+    assert orth <= 1e-13, orth
+    res = residual(backend, Y, R, G)
+    assert res <= 1e-13, res
+    # S - S^H = 2 i Im(d) Q^H Q for the orthonormal Q.
+    S = F.project(Y, G)
+    err = float(np.linalg.norm(S - S.conj().T - 2j * D.imag * np.eye(K)) / np.linalg.norm(S))
+    assert err <= 1e-13, err
+
+
+def test_mixed_cholqr2_meets_the_orthogonality_bar(backend, basis):
+    plan, G, omega, Y = basis
+    print("full size: mixed cholqr2", flush=True)
+    for solve in ("fp32", "fp64"):
+        F.panelmul(Y, G, omega)
+        R = F.cholqr2_mixed(Y, solve=solve)
+        orth = float(np.linalg.norm(F.gram(Y) - np.eye(K)))
+        assert orth <= 1e-13, (solve, orth)
+        res = residual(backend, Y, R, G)
+        # The FP32 solve leaves the factorization at FP32 level; the FP64
+        # solve keeps it at FP64 level.
+        assert res <= (1e-5 if solve == "fp32" else 1e-13), (solve, res)
