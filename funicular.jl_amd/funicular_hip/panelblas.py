@@ -98,11 +98,7 @@ def cholqr2(Y, *, nbuffers=None):
     triangular R with Q R equal to the original Y (src/panelblas.jl:102-139).
     A pass whose Gram needed a shift triggers a third pass (shifted
     CholeskyQR3, Fukaya et al.)."""
-    R, shifted = cholqr_pass(Y, nbuffers)
-    R, shifted2 = cholqr_pass_then(Y, R, nbuffers)
-    if shifted or shifted2:
-        R, _ = cholqr_pass_then(Y, R, nbuffers)
-    return R
+    return _passes(Y, nbuffers)
 
 
 def triangular_product(A, B):
@@ -117,7 +113,7 @@ def triangular_product(A, B):
 _product_pool = None
 
 
-def _device_pass(Y, nbuffers, R_prev=None, first=None):
+def _device_pass(Y, nbuffers, R_prev=None, first=None, gram_ready=None, fuse_next=None):
     """One CholQR pass with G, R and the solve's factor kept on the device
     (src/panelblas.jl:141-146): the Gram sweep into a device accumulator, its
     plain Cholesky factorization on the device, the solve sweep from the
@@ -127,19 +123,30 @@ def _device_pass(Y, nbuffers, R_prev=None, first=None):
     reference's rule takes over there (a plain retry, then the shift).
 
     first=(shift, solve) makes it the mixed first pass of cholqr2_mixed: FP32
-    Gram and Cholesky, the solve in FP32 or FP64. Returns (R_total, shifted)
-    for an FP64 pass and (R, s) for the mixed one."""
+    Gram and Cholesky, the solve in FP32 or FP64.
+
+    gram_ready is a device accumulator that already holds this pass's Gram
+    (the previous pass's solve sweep formed it); the pass takes it over.
+    fuse_next(shifted) says, once this pass's Cholesky is known, whether its
+    solve sweep should form the next pass's Gram as well, which saves that
+    pass a full sweep over Y (a full upload of it when Y lives in host
+    memory). Returns (R_total, shifted, next_gram) for an FP64 pass and
+    (R, s, next_gram) for the mixed one."""
     global _product_pool
     plan = Y.plan
     backend = plan.backend
     T = Y.dtype
     k = Y.k
-    acc, factor = _plan.checkout_device_buffers(plan, T, (k, k), 2)
-    info = narrow = None
+    acc = gram_ready
+    factor = info = narrow = nxt = work_g = None
     try:
+        if acc is None:
+            acc = _plan.checkout_device_buffers(plan, T, (k, k), 1)[0]
+        factor = _plan.checkout_device_buffers(plan, T, (k, k), 1)[0]
         info = _plan.checkout_device_buffers(plan, np.int32, (1, 1), 1)[0]
         compute = None if first is None else np.complex64
-        gram_into(acc, Y, Y, nbuffers=nbuffers, compute=compute)
+        if gram_ready is None:
+            gram_into(acc, Y, Y, nbuffers=nbuffers, compute=compute)
         shifted, s = False, 0.0
         bad = 1
         if first is None:
@@ -169,25 +176,61 @@ def _device_pass(Y, nbuffers, R_prev=None, first=None):
                                                                       thread_name_prefix="cholqr-product")
             product = _product_pool.submit(triangular_product, R, R_prev)
         solve = None if first is None or first[1] == "fp64" else np.complex64
+        if fuse_next is not None and fuse_next(shifted or s > 0.0):
+            traversal = rowtraversal(Y)
+            height = traversal.blocks[0][1] - traversal.blocks[0][0]
+            nxt = _plan.checkout_device_buffers(plan, T, (k, k), 1)[0]
+            work_g = _plan.checkout_workspace(plan, backend.gram_workspace_bytes(height, k, k, T, hermitian=True))
         try:
-            rdiv_rows(Y, R, nbuffers=nbuffers, compute=solve, factor=factor)
+            rdiv_rows(Y, R, nbuffers=nbuffers, compute=solve, factor=factor,
+                      next_gram=None if nxt is None else (nxt, work_g))
         finally:
             total = R if product is None else product.result()
-        return (total, shifted) if first is None else (R, s)
+        handed, nxt = nxt, None
+        return ((total, shifted) if first is None else (R, s)) + (handed,)
     finally:
-        _plan.checkin_device_buffers(plan, [acc, factor, info, narrow])
+        _plan.checkin_device_buffers(plan, [acc, factor, info, narrow, nxt, work_g])
 
 
 def cholqr_pass_then(Y, R_prev, nbuffers=None):
     """One more CholQR pass over Y, returning (R_pass R_prev, shifted). The
     host forms the triangular product while the pass's solve sweep runs on the
     device, instead of after it."""
-    return _device_pass(Y, nbuffers, R_prev=R_prev)
+    return _device_pass(Y, nbuffers, R_prev=R_prev)[:2]
 
 
 def cholqr_pass(Y, nbuffers=None):
     """src/panelblas.jl:141-146."""
-    return _device_pass(Y, nbuffers)
+    return _device_pass(Y, nbuffers)[:2]
+
+
+def _plan_stores_compute_eltype(Y):
+    plan = Y.plan
+    T = np.dtype(Y.dtype)
+    store = getattr(Y, "store", None)
+    return ((plan.host_eltype is None or np.dtype(plan.host_eltype) == T)
+            and (plan.disk_eltype is None or np.dtype(plan.disk_eltype) == T)
+            and (store is None or np.dtype(store.stored) == T))
+
+
+def _passes(Y, nbuffers, first=None):
+    """The passes of cholqr2 (first=None) or cholqr2_mixed: the first pass,
+    the FP64 pass, and a third FP64 pass when a Cholesky needed a shift
+    (src/panelblas.jl:130-139). Off HBM each solve sweep also forms the next
+    pass's Gram (the same products in the same order, so the same bits), and
+    only when a next pass will run: after pass two, that is known from the two
+    Cholesky factorizations before its solve starts."""
+    # The fused Gram sees each block as solved, before any narrowing on its
+    # way back, so it is taken only where the tiers store the compute eltype.
+    fuse = (not Y.resident and _plan_stores_compute_eltype(Y))
+    R, shift1, g = _device_pass(Y, nbuffers, first=first, fuse_next=(lambda _: True) if fuse else None)
+    shifted1 = shift1 > 0.0 if first is not None else shift1
+    # A pass given gram_ready owns it from then on, and checks it back in.
+    R, shifted2, g2 = _device_pass(Y, nbuffers, R_prev=R, gram_ready=g,
+                                   fuse_next=(lambda s2: shifted1 or s2) if fuse else None)
+    if shifted1 or shifted2:
+        R, _, _ = _device_pass(Y, nbuffers, R_prev=R, gram_ready=g2)
+    return R
 
 
 def _potrf(G):
@@ -220,9 +263,12 @@ def cholesky_upper(G, N, k):
     return shifted, True
 
 
-def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None):
+def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None, next_gram=None):
     """Y <- Y R^-1 over row blocks (src/panelblas.jl:157-169). `factor`, a
-    device copy of R already in place, saves the upload."""
+    device copy of R already in place, saves the upload. next_gram=(acc,
+    work) also accumulates the Gram of the solved blocks into the device
+    matrix acc while each block is still on the device: the next CholQR
+    pass's Gram, without a sweep of its own."""
     plan = Y.plan
     T = Y.dtype
     backend = plan.backend
@@ -241,10 +287,13 @@ def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None):
                 backend.rdiv_upper(Yb, factor, work, compute)
             else:
                 backend.rdiv_upper_folded(Yb, work, compute)
+            if next_gram is not None:
+                backend.gram_accumulate(next_gram[0], Yb, Yb, 1.0, carry(block), next_gram[1])
         # Rows are independent, so a matrix resident in HBM needs no row
         # blocks: one launch over all N rows leaves no partly filled last wave
-        # of workgroups per block (3% of a 629,856-row launch).
-        traversal = RowSteps(Y.N, Y.N) if Y.resident else rowtraversal(Y)
+        # of workgroups per block (3% of a 629,856-row launch). A Gram rides
+        # along only on row blocks, its chunking being that of gram's sweep.
+        traversal = RowSteps(Y.N, Y.N) if (Y.resident and next_gram is None) else rowtraversal(Y)
         sweep(step, updatepanels(Y), nbuffers=nbuffers, traversal=traversal)
     finally:
         _plan.checkin_device_buffers(plan, [factor, work] if owned else [work])
@@ -616,7 +665,7 @@ def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp32"):
         raise ArgumentError(f"solve must be 'fp64' or 'fp32', got {solve!r}")
     if shift not in ("adaptive", "fukaya"):
         raise ArgumentError(f"shift must be 'adaptive' or 'fukaya', got {shift!r}")
-    return _device_pass(Y, nbuffers, first=(shift, solve))
+    return _device_pass(Y, nbuffers, first=(shift, solve))[:2]
 
 
 def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32"):
@@ -626,8 +675,10 @@ def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32"):
     shift, as shifted CholeskyQR3 prescribes: a shifted first pass leaves Q1
     with a condition number well above 1, which one FP64 pass does not bring
     to FP64 orthogonality."""
-    R1, s = cholqr_pass_mixed(Y, nbuffers, shift, solve)
-    R, shifted = cholqr_pass_then(Y, R1, nbuffers)
-    if shifted or s > 0.0:
-        R, _ = cholqr_pass_then(Y, R, nbuffers)
-    return R
+    if Y.dtype != np.dtype(np.complex128):
+        raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
+    if solve not in ("fp64", "fp32"):
+        raise ArgumentError(f"solve must be 'fp64' or 'fp32', got {solve!r}")
+    if shift not in ("adaptive", "fukaya"):
+        raise ArgumentError(f"shift must be 'adaptive' or 'fukaya', got {shift!r}")
+    return _passes(Y, nbuffers, first=(shift, solve))

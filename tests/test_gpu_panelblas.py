@@ -233,6 +233,30 @@ def test_device_resident_plan(backend, nb):
     assert np.array_equal(Q, F.to_matrix(Yh))
 
 
+@pytest.mark.parametrize("mixed", (False, True))
+def test_streamed_cholqr2_uploads_y_three_times(backend, mixed):
+    # Off HBM each solve sweep also forms the next pass's Gram, so cholqr2
+    # uploads Y three times (Gram, solve + Gram, solve) instead of four, and
+    # the bits match the resident run, whose passes are not fused.
+    rng = np.random.default_rng(501)
+    N, k, w = 3000, 96, 16
+    A = randmatrix(rng, N, k)
+    run = (lambda Y: F.cholqr2_mixed(Y)) if mixed else (lambda Y: F.cholqr2(Y))
+    hp = make_plan(backend, device_budget=256 * 2 ** 20)
+    Yh = F.PanelMatrix.from_array(A, plan=hp, w=w)
+    c0 = F._lib.counters()
+    Rh = run(Yh)
+    c1 = F._lib.counters()
+    up = c1["h2d_bytes"] - c0["h2d_bytes"]
+    assert 3 * N * k * 16 <= up < 3 * N * k * 16 + 8 * k * k * 16, up
+    rp = make_plan(backend, device_budget=256 * 2 ** 20, device_resident=True)
+    Yr = F.PanelMatrix.from_array(A, plan=rp, w=w)
+    Rr = run(Yr)
+    assert np.array_equal(Rh, Rr)
+    assert np.array_equal(F.to_matrix(Yh), F.to_matrix(Yr))
+    assert OB.orthogonality_error(F.to_matrix(Yh)) <= 1e-13
+
+
 @pytest.mark.parametrize("w", (23, 8, 1))
 def test_project_on_a_device_resident_plan(backend, w):
     # With room for a second N x k buffer, G Q is formed whole and S is one
