@@ -138,7 +138,7 @@ def _device_pass(Y, nbuffers, R_prev=None, first=None, gram_ready=None, fuse_nex
     T = Y.dtype
     k = Y.k
     acc = gram_ready
-    factor = info = narrow = nxt = work_g = None
+    factor = info = narrow = nxt = work_g = rounding = None
     try:
         if acc is None:
             acc = _plan.checkout_device_buffers(plan, T, (k, k), 1)[0]
@@ -176,20 +176,24 @@ def _device_pass(Y, nbuffers, R_prev=None, first=None, gram_ready=None, fuse_nex
                                                                       thread_name_prefix="cholqr-product")
             product = _product_pool.submit(triangular_product, R, R_prev)
         solve = None if first is None or first[1] == "fp64" else np.complex64
+        rounding = None
         if fuse_next is not None and fuse_next(shifted or s > 0.0):
             traversal = rowtraversal(Y)
             height = traversal.blocks[0][1] - traversal.blocks[0][0]
             nxt = _plan.checkout_device_buffers(plan, T, (k, k), 1)[0]
             work_g = _plan.checkout_workspace(plan, backend.gram_workspace_bytes(height, k, k, T, hermitian=True))
+            narrow_to = _fusable(Y)[1]
+            if narrow_to is not None:
+                rounding = _plan.checkout_device_buffers(plan, narrow_to, (height, k), 1)[0]
         try:
             rdiv_rows(Y, R, nbuffers=nbuffers, compute=solve, factor=factor,
-                      next_gram=None if nxt is None else (nxt, work_g))
+                      next_gram=None if nxt is None else (nxt, work_g, rounding))
         finally:
             total = R if product is None else product.result()
         handed, nxt = nxt, None
         return ((total, shifted) if first is None else (R, s)) + (handed,)
     finally:
-        _plan.checkin_device_buffers(plan, [acc, factor, info, narrow, nxt, work_g])
+        _plan.checkin_device_buffers(plan, [acc, factor, info, narrow, nxt, work_g, rounding])
 
 
 def cholqr_pass_then(Y, R_prev, nbuffers=None):
@@ -204,13 +208,22 @@ def cholqr_pass(Y, nbuffers=None):
     return _device_pass(Y, nbuffers)[:2]
 
 
-def _plan_stores_compute_eltype(Y):
+def _fusable(Y):
+    """Whether the solve sweeps of Y may form the next pass's Gram, and the
+    eltype a solved block is rounded to first: that of the host tier when it
+    narrows, so the Gram sees exactly the values the next pass would read
+    back; None when nothing narrows. A file or disk tier that narrows on its
+    own rules fusion out."""
     plan = Y.plan
     T = np.dtype(Y.dtype)
     store = getattr(Y, "store", None)
-    return ((plan.host_eltype is None or np.dtype(plan.host_eltype) == T)
-            and (plan.disk_eltype is None or np.dtype(plan.disk_eltype) == T)
-            and (store is None or np.dtype(store.stored) == T))
+    if plan.disk_eltype is not None and np.dtype(plan.disk_eltype) != T:
+        return False, None
+    if store is not None and np.dtype(store.stored) != T:
+        return False, None
+    if plan.host_eltype is not None and np.dtype(plan.host_eltype) != T:
+        return True, np.dtype(plan.host_eltype)
+    return True, None
 
 
 def _passes(Y, nbuffers, first=None):
@@ -220,9 +233,8 @@ def _passes(Y, nbuffers, first=None):
     pass's Gram (the same products in the same order, so the same bits), and
     only when a next pass will run: after pass two, that is known from the two
     Cholesky factorizations before its solve starts."""
-    # The fused Gram sees each block as solved, before any narrowing on its
-    # way back, so it is taken only where the tiers store the compute eltype.
-    fuse = (not Y.resident and _plan_stores_compute_eltype(Y))
+    fusable, _ = _fusable(Y)
+    fuse = not Y.resident and fusable
     R, shift1, g = _device_pass(Y, nbuffers, first=first, fuse_next=(lambda _: True) if fuse else None)
     shifted1 = shift1 > 0.0 if first is not None else shift1
     # A pass given gram_ready owns it from then on, and checks it back in.
@@ -268,7 +280,9 @@ def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None, next_gram=None)
     device copy of R already in place, saves the upload. next_gram=(acc,
     work) also accumulates the Gram of the solved blocks into the device
     matrix acc while each block is still on the device: the next CholQR
-    pass's Gram, without a sweep of its own."""
+    pass's Gram, without a sweep of its own; next_gram=(acc, work, rounding)
+    with a device buffer `rounding` of the host tier's narrower eltype first
+    rounds each solved block to it, as storing it would."""
     plan = Y.plan
     T = Y.dtype
     backend = plan.backend
@@ -288,7 +302,14 @@ def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None, next_gram=None)
             else:
                 backend.rdiv_upper_folded(Yb, work, compute)
             if next_gram is not None:
-                backend.gram_accumulate(next_gram[0], Yb, Yb, 1.0, carry(block), next_gram[1])
+                acc, gwork, rounding = next_gram
+                if rounding is not None:
+                    # Round the solved block to the host eltype it is stored in,
+                    # so the Gram is that of the values the next pass reads.
+                    r = rounding.view((0, Yb.rows), None)
+                    backend.convert_copy(r, Yb)
+                    backend.convert_copy(Yb, r)
+                backend.gram_accumulate(acc, Yb, Yb, 1.0, carry(block), gwork)
         # Rows are independent, so a matrix resident in HBM needs no row
         # blocks: one launch over all N rows leaves no partly filled last wave
         # of workgroups per block (3% of a 629,856-row launch). A Gram rides

@@ -257,6 +257,29 @@ def test_streamed_cholqr2_uploads_y_three_times(backend, mixed):
     assert OB.orthogonality_error(F.to_matrix(Yh)) <= 1e-13
 
 
+def test_fused_passes_round_to_a_narrower_host_tier(backend):
+    # With ComplexF32 host storage the fused Gram is formed after rounding
+    # each solved block to ComplexF32, so cholqr2 matches the unfused passes
+    # (a pass, then a pass on what was stored) to the bit.
+    from funicular_hip.panelblas import cholqr_pass, cholqr_pass_then
+    rng = np.random.default_rng(502)
+    N, k, w = 3000, 64, 16
+    A = randmatrix(rng, N, k)
+    plan = make_plan(backend, device_budget=256 * 2 ** 20, host_eltype=np.complex64)
+    Y1 = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    c0 = F._lib.counters()
+    R1 = F.cholqr2(Y1)
+    c1 = F._lib.counters()
+    up = c1["h2d_bytes"] - c0["h2d_bytes"]
+    assert 3 * N * k * 8 <= up < 3 * N * k * 8 + 8 * k * k * 16, up
+    Y2 = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    R2, s1 = cholqr_pass(Y2)
+    R2, s2 = cholqr_pass_then(Y2, R2)
+    assert not (s1 or s2)
+    assert np.array_equal(R1, R2)
+    assert np.array_equal(F.to_matrix(Y1), F.to_matrix(Y2))
+
+
 @pytest.mark.parametrize("w", (23, 8, 1))
 def test_project_on_a_device_resident_plan(backend, w):
     # With room for a second N x k buffer, G Q is formed whole and S is one
