@@ -226,6 +226,29 @@ function panel_reduce!(::HIPBackend, acc::HIPMatrix{ComplexF64}, Xb::HIPMatrix{T
     acc
 end
 
+# With a `factor_gram!(b, R, G, N) -> shifted` hook in cholqr_pass!
+# (src/panelblas.jl:141-146; its default copies G to the host and runs
+# cholesky_upper), the plain factorization runs on the device; a bad pivot
+# hands G to cholesky_upper on the host, which retries and shifts as the
+# reference does.
+function factor_gram!(b::HIPBackend, R::HIPMatrix{T}, G::HIPMatrix{T}, N::Integer) where {T}
+    info = alloc_device(b, Cint, (1, 1))
+    check(ccall((:fhip_potrf_upper, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Cint, Ptr{Cvoid}, Int64, Cint, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
+        R.ptr, R.ld, eltcode(T), G.ptr, G.ld, eltcode(T), size(G, 1), info.ptr, current_stream()))
+    bad = Ref{Cint}(0)
+    check(ccall((:fhip_memcpy2d_async, LIB[]), Cint,
+        (Ptr{Cvoid}, Csize_t, Ptr{Cvoid}, Csize_t, Csize_t, Csize_t, Cint, Ptr{Cvoid}),
+        bad, sizeof(Cint), info.ptr, sizeof(Cint), sizeof(Cint), 1, FHIP_D2H, current_stream()))
+    check(ccall((:fhip_stream_sync, LIB[]), Cint, (Ptr{Cvoid},), current_stream()))
+    bad[] == 0 && return false
+    G_host = Array{T}(undef, size(G))
+    copy2d!(G_host, G, FHIP_D2H, nothing)
+    R_host, shifted = Funicular.cholesky_upper(G_host, N, size(G, 1))  # the reference's retry and shift
+    copy2d!(R, R_host, FHIP_H2D, nothing)
+    shifted
+end
+
 # A per-backend workspace that grows to the largest request and is reused;
 # the first sweep of a shape allocates it, later ones do not.
 const WORK = Ref{Any}(nothing)
