@@ -20,6 +20,7 @@ import Funicular: alloc_device, alloc_host_slab, d2h!, devicetype, h2d!, hip_bac
                   make_queue, make_sweep_queues, record_event, submit!, sync_all, sync_event, sync_queue,
                   wait_event, supports_eltype, gram_accumulate!, rightmul_gemm!, rdiv_upper!, convert_copy!
 
+import Funicular
 using Funicular: DeviceBackend, SLAB_ALIGN, aligned_slab
 using LinearAlgebra
 
