@@ -447,120 +447,142 @@ def project(Q, G, *, nbuffers=None):
 
 # CholeskyQR by column panels, for a basis that the host budget cannot hold
 # whole. A row sweep stages every panel at once (src/executor.jl:66-67), so
-# gram and cholqr2 refuse such a matrix; these sweeps hold one panel on the
-# device while the others stream past it, so only the staging buffers need to
-# fit. The reference has no such traversal.
+# gram and cholqr2 refuse such a matrix; these sweeps hold a run of panels on
+# the device while the others stream past it, so only the run and the
+# staging buffers need to fit. The reference has no such traversal.
 
-def panel_schedule_gram(p):
-    """Steps of gram_panelwise: for each panel j, (True, j, j) stages panel j
-    and forms its diagonal block; (False, i, j), i < j, then forms block (i, j)
-    against the held copy of panel j. p (p + 1) / 2 steps."""
-    panels, labels = [], []
-    for j in range(p):
-        panels.append(j)
-        labels.append((True, j, j))
-        for i in range(j):
-            panels.append(i)
-            labels.append((False, i, j))
-    return panels, labels
+def panel_group(X, nbuffers=None):
+    """How many consecutive column panels the panel-wise sweeps hold on the
+    device at once: as many as the plan's device pool can still lend beside
+    one sweep's staging buffers, and at least one."""
+    plan = X.plan
+    pool = plan.devicepool
+    panel = X.N * X.w * np.dtype(X.dtype).itemsize
+    nb = plan.nbuffers if nbuffers is None else int(nbuffers)
+    spare = pool.budget - pool.allocated - (nb + 2) * panel - (64 << 20)
+    return int(max(1, min(npanels(X), spare // max(panel, 1))))
 
 
-def gram_panelwise(X, *, nbuffers=None, compute=None):
-    """The Hermitian k x k Gram X^H X from pairs of column panels: block (i, j)
-    of the upper triangle is X_i^H X_j, formed with panel j held on the device.
-    The lower triangle is mirrored on the host. compute=np.complex64 forms the
+def _groups(X, g):
+    """(j0, j1, c0, c1) for each run of g consecutive panels: panels j0..j1-1,
+    columns c0..c1-1."""
+    p = npanels(X)
+    for j0 in range(0, p, g):
+        j1 = min(p, j0 + g)
+        yield j0, j1, panelrange(X, j0).start, panelrange(X, j1 - 1).stop
+
+
+def _hold_group(X, T, gcols):
+    plan = X.plan
+    return _plan.checkout_device_buffers(plan, T, (X.N, gcols), 1)[0]
+
+
+def gram_panelwise(X, *, nbuffers=None, compute=None, group=None):
+    """The Hermitian k x k Gram X^H X by column panels, for a matrix the row
+    sweeps cannot stage whole. A run of g consecutive panels (g = `group`, by
+    default panel_group's) is copied into one device buffer and its own block
+    of G is one Hermitian Gram; then every earlier panel streams past it once
+    for its blocks against the run. Each panel is read 1 + (number of runs
+    after it) times: p (p + 1) / 2 reads in all for g = 1, p for g = p. The
+    lower triangle is mirrored on the host. compute=np.complex64 forms the
     blocks on the FP32 matrix cores, as gram does."""
     plan = X.plan
     backend = plan.backend
     T = X.dtype
-    p = npanels(X)
-    panels, labels = panel_schedule_gram(p)
-    hold = _plan.checkout_device_buffers(plan, T, (X.N, X.w), 1)[0]
+    g = panel_group(X, nbuffers) if group is None else max(1, int(group))
+    gmax = max(c1 - c0 for _, _, c0, c1 in _groups(X, g))
+    hold = _hold_group(X, T, gmax)
     acc = work = None
     try:
         acc = _plan.checkout_device_buffers(plan, T, (X.k, X.k), 1)[0]
-        need = max(backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=True, compute=compute),
-                   backend.gram_workspace_bytes(X.N, X.w, X.w, T, hermitian=False, compute=compute))
+        need = max(backend.gram_workspace_bytes(X.N, gmax, gmax, T, hermitian=True, compute=compute),
+                   backend.gram_workspace_bytes(X.N, X.w, gmax, T, hermitian=False, compute=compute))
         work = _plan.checkout_workspace(plan, need)
+        for j0, j1, c0, c1 in _groups(X, g):
+            held = hold.view(None, (0, c1 - c0))
 
-        def step(label, Xb):
-            first, i, j = label
-            rj = panelrange(X, j)
-            held = hold.view(None, (0, panelwidth(X, j)))
-            if first:
-                backend.convert_copy(held, Xb)
-                backend.gram_accumulate(acc.view((rj.start, rj.stop), (rj.start, rj.stop)), Xb, Xb, 1.0, 0.0, work,
-                                        compute)
-            else:
-                ri = panelrange(X, i)
-                backend.gram_accumulate(acc.view((ri.start, ri.stop), (rj.start, rj.stop)), Xb, held, 1.0, 0.0, work,
-                                        compute)
-        sweep(step, readpanels(X), nbuffers=nbuffers, traversal=PanelSteps(panels, labels))
+            # The run's panels into the device buffer; its own block once the
+            # last one is in (same compute stream).
+            def load(j, Xb, j1=j1, c0=c0, c1=c1, held=held):
+                r = panelrange(X, j)
+                backend.convert_copy(held.view(None, (r.start - c0, r.stop - c0)), Xb)
+                if j == j1 - 1:
+                    backend.gram_accumulate(acc.view((c0, c1), (c0, c1)), held, held, 1.0, 0.0, work, compute)
+            sweep(load, readpanels(X), nbuffers=nbuffers, traversal=PanelSteps(list(range(j0, j1))))
+            if j0 > 0:
+                def cross(i, Xb, c0=c0, c1=c1, held=held):
+                    r = panelrange(X, i)
+                    backend.gram_accumulate(acc.view((r.start, r.stop), (c0, c1)), Xb, held, 1.0, 0.0, work, compute)
+                sweep(cross, readpanels(X), nbuffers=nbuffers, traversal=PanelSteps(list(range(j0))))
         G = collect_small(acc, plan)
     finally:
         _plan.checkin_device_buffers(plan, [hold, acc, work])
     return np.triu(G) + np.triu(G, 1).conj().T
 
 
-def rdiv_panelwise(Y, R, *, nbuffers=None):
-    """Y <- Y R^-1 column panel by column panel, left looking:
-    Y_j <- (Y_j - sum_{i<j} Y_i R_ij) R_jj^-1, the panels before j already
-    solved. Panel j is held on the device while the solved panels stream past
-    it, then solved and written back."""
+def rdiv_panelwise(Y, R, *, nbuffers=None, group=None):
+    """Y <- Y R^-1 by column panels, left looking over runs of g consecutive
+    panels (g = `group`, by default panel_group's): a run is copied into one
+    device buffer, every solved panel before it streams past once
+    (Y_run <- Y_run - Q_i R_i,run, on the scaled row-block product), the run
+    is solved by its own diagonal block of R (the solve kernel over the run's
+    columns), and its panels are written back."""
     plan = Y.plan
     backend = plan.backend
     T = Y.dtype
-    p = npanels(Y)
-    hold = _plan.checkout_device_buffers(plan, T, (Y.N, Y.w), 1)[0]
+    g = panel_group(Y, nbuffers) if group is None else max(1, int(group))
+    gmax = max(c1 - c0 for _, _, c0, c1 in _groups(Y, g))
+    hold = _hold_group(Y, T, gmax)
     factor = work = None
     try:
         factor = _plan.checkout_device_buffers(plan, T, R.shape, 1)[0]
         backend.h2d(factor, np.asfortranarray(R.astype(T, copy=False)))
-        work = _plan.checkout_workspace(plan, backend.trsm_workspace_bytes(Y.w, T))
-        for j in range(p):
-            rj = panelrange(Y, j)
-            held = hold.view(None, (0, panelwidth(Y, j)))
+        work = _plan.checkout_workspace(plan, backend.trsm_workspace_bytes(gmax, T))
+        for j0, j1, c0, c1 in _groups(Y, g):
+            held = hold.view(None, (0, c1 - c0))
 
-            def update(label, Yb, rj=rj, held=held):
-                first, i = label
-                if first:
-                    backend.convert_copy(held, Yb)
-                else:
-                    ri = panelrange(Y, i)
-                    backend.rightmul_gemm_scaled(held, Yb, factor.view((ri.start, ri.stop), (rj.start, rj.stop)),
-                                                 -1.0, 1.0)
-            sweep(update, readpanels(Y), nbuffers=nbuffers,
-                  traversal=PanelSteps([j] + list(range(j)), [(True, j)] + [(False, i) for i in range(j)]))
+            def load(j, Yb, c0=c0, held=held):
+                r = panelrange(Y, j)
+                backend.convert_copy(held.view(None, (r.start - c0, r.stop - c0)), Yb)
+            sweep(load, readpanels(Y), nbuffers=nbuffers, traversal=PanelSteps(list(range(j0, j1))))
+            if j0 > 0:
+                def update(i, Yb, c0=c0, c1=c1, held=held):
+                    r = panelrange(Y, i)
+                    backend.rightmul_gemm_scaled(held, Yb, factor.view((r.start, r.stop), (c0, c1)), -1.0, 1.0)
+                sweep(update, readpanels(Y), nbuffers=nbuffers, traversal=PanelSteps(list(range(j0))))
 
-            def solve(_, Yb, rj=rj, held=held):
-                backend.rdiv_upper(held, factor.view((rj.start, rj.stop), (rj.start, rj.stop)), work)
-                backend.convert_copy(Yb, held)
-            sweep(solve, writepanels(Y), nbuffers=1, traversal=PanelSteps([j]))
+            def solve(j, Yb, j0=j0, c0=c0, c1=c1, held=held):
+                if j == j0:
+                    backend.rdiv_upper(held, factor.view((c0, c1), (c0, c1)), work)
+                r = panelrange(Y, j)
+                backend.convert_copy(Yb, held.view(None, (r.start - c0, r.stop - c0)))
+            sweep(solve, writepanels(Y), nbuffers=nbuffers, traversal=PanelSteps(list(range(j0, j1))))
     finally:
         _plan.checkin_device_buffers(plan, [hold, factor, work])
     return Y
 
 
-def cholqr2_panelwise(Y, *, nbuffers=None, mixed=False, shift="adaptive"):
+def cholqr2_panelwise(Y, *, nbuffers=None, mixed=False, shift="adaptive", group=None):
     """cholqr2 by column panels (gram_panelwise, host Cholesky,
     rdiv_panelwise), for a basis whose panels do not all fit the host budget.
     Two passes and a third when a Cholesky needed a shift, as cholqr2.
     mixed=True makes the first pass the mixed one of cholqr2_mixed with
     solve="fp64": its Gram on the FP32 matrix cores and its Cholesky in FP32
     (shifted as `shift` says), the solve in FP64. Returns R with Q R equal to
-    the original Y; Y holds Q."""
+    the original Y; Y holds Q. `group` is the run length of the panel-wise
+    sweeps (panel_group's by default)."""
     def one_pass():
-        G = gram_panelwise(Y, nbuffers=nbuffers)
+        G = gram_panelwise(Y, nbuffers=nbuffers, group=group)
         R, shifted = cholesky_upper(G, Y.N, Y.k)
-        rdiv_panelwise(Y, R, nbuffers=nbuffers)
+        rdiv_panelwise(Y, R, nbuffers=nbuffers, group=group)
         return R, shifted
     if mixed:
         if Y.dtype != np.dtype(np.complex128):
             raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
-        G = gram_panelwise(Y, nbuffers=nbuffers, compute=np.complex64)
+        G = gram_panelwise(Y, nbuffers=nbuffers, compute=np.complex64, group=group)
         R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
         R = R32.astype(Y.dtype)
-        rdiv_panelwise(Y, R, nbuffers=nbuffers)
+        rdiv_panelwise(Y, R, nbuffers=nbuffers, group=group)
         shifted = s > 0.0
     else:
         R, shifted = one_pass()

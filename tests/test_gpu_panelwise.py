@@ -40,11 +40,14 @@ def test_panelwise_against_the_oracle(backend, T, w, nb):
     A = randmatrix(rng, N, k, T)
     plan = make_plan(backend)
     tol = OB.blastol(T, N, k)
-    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
-    assert OB.isapprox(F.gram_panelwise(X, nbuffers=nb), OB.gram_panelwise(A, w), tol)
     R, _ = OB.cholesky_upper(OB.gram(A), N, k)
-    F.rdiv_panelwise(X, R, nbuffers=nb)
-    assert OB.isapprox(F.to_matrix(X), OB.rdiv_panelwise(A, R, w), tol)
+    # group=1 is the pairwise schedule (one panel held); 3 leaves a ragged
+    # last run; None holds as many panels as the device budget allows.
+    for group in (1, 3, None):
+        X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+        assert OB.isapprox(F.gram_panelwise(X, nbuffers=nb, group=group), OB.gram_panelwise(A, w), tol)
+        F.rdiv_panelwise(X, R, nbuffers=nb, group=group)
+        assert OB.isapprox(F.to_matrix(X), OB.rdiv_panelwise(A, R, w), tol)
     Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
     Rq = F.cholqr2_panelwise(Y, nbuffers=nb)
     Qo, Ro = OB.cholqr2_panelwise(A, w)
@@ -98,7 +101,7 @@ def test_panelwise_cholqr2_of_a_basis_past_the_host_budget(backend, tmp_path):
     assert all(p.pins == 0 for p in Y.panels)
     assert np.array_equal(F.to_matrix(Y), A)
     before = F.disk_reads(Y)
-    R = F.cholqr2_panelwise(Y)
+    R = F.cholqr2_panelwise(Y, group=3)
     assert F.disk_reads(Y) > before and F.disk_writes(Y) > 0
     assert all(p.pins == 0 for p in Y.panels)
     Qo, Ro = OB.cholqr2(A, w)
