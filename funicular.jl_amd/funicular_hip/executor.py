@@ -293,6 +293,7 @@ def sweep(f, *operands, nbuffers=None, traversal=None):
     up, compute, down = backend.make_sweep_queues()
     slotfree = [None] * nb
     feed = stage_channel(steps, operands)
+    failed = False
     try:
         for _ in range(count):
             step = feed.take()
@@ -332,11 +333,14 @@ def sweep(f, *operands, nbuffers=None, traversal=None):
         backend.sync_queue(up)
         backend.sync_queue(down)
     except BaseException:
-        clear_pins(operands)
+        failed = True
         raise
     finally:
         feed.close()
         sync_quietly(backend, (up, compute, down))
+        if failed:
+            # Once the producer has stopped, so that it pins nothing more.
+            clear_pins(operands)
         for b in buffers:
             if b is not None:
                 _plan.checkin_device_buffers(plan, b)

@@ -113,3 +113,29 @@ def test_narrowed_disk_tier_feeds_the_device(backend, tmp_path):
     assert F.disk_reads(X) > 0
     B32 = B.astype(S).astype(C16)
     assert OB.isapprox(F.gram(Z), B32.conj().T @ B32, 1e-12)
+
+
+def test_failed_write_only_sweep_keeps_the_old_panels(backend, tmp_path):
+    rng = np.random.default_rng(76)
+    N, k, w = 300, 24, 4
+    A, B = randmatrix(rng, N, k), randmatrix(rng, N, k)
+    plan = make_plan(backend, tmp_path, 8, N, w)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Y = F.PanelMatrix.from_array(B, plan=plan, w=w)
+    for p in Y.panels:
+        F.evict(p, plan)
+
+    def copy_then_fail(j, xb, yb):
+        if j == 2:
+            raise RuntimeError("panel function failed")
+        backend.d2d(yb, xb)
+
+    with pytest.raises(RuntimeError, match="panel function failed"):
+        F.sweep(copy_then_fail, F.readpanels(X), F.writepanels(Y), nbuffers=2)
+    assert all(p.pins == 0 for p in Y.panels)
+    # Panels 0 and 1 were written; the rest hold what they held, including the
+    # ones the producer had staged ahead of the failure.
+    expect = B.copy()
+    expect[:, :2 * w] = A[:, :2 * w]
+    assert np.array_equal(F.to_matrix(Y), expect)
+    assert np.array_equal(F.to_matrix(X), A)
