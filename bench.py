@@ -583,6 +583,18 @@ def kernels_only(rows, k, reps, vendor=False):
     timed("norm_fp64", lambda: backend.dot_accumulate(acc, A, None, False, work), 0, blk)
     timed("scale_fp64", lambda: backend.scale(B, 0.6 + 0.8j), 0, 2 * blk)
     timed("axpby_fp64", lambda: backend.axpby(B, A, 0.5, 1.0), 0, 3 * blk)
+    # The stencils on one panel of their workloads (HBM-bound; ideal bytes:
+    # the panel read once and the image written once): the 216^3 Helmholtz
+    # operator on 32 columns and the 2^22-row tridiagonal on 32 columns.
+    for name, Gop, n_rows in (("helmholtz_fp64", F.Helmholtz3D(216, 216, 216, 6.0 - 0.09 * (1 + 0.05j), -1.0,
+                                                                backend=backend), 216 ** 3),
+                              ("tridiag_fp64", F.Tridiagonal1D(2 ** 22, 2 + 0.1j, -1.0, backend=backend), 2 ** 22)):
+        P = backend.alloc_device(np.complex128, (n_rows, 32))
+        Q = backend.alloc_device(np.complex128, (n_rows, 32))
+        backend.ghost_fill(P, 0, 1, 7)
+        timed(name, lambda: Gop.apply(Q, P), 0, 2 * n_rows * 32 * 16)
+        backend.sync_all()
+        del P, Q
     backend.sync_all()
     if vendor:
         vendor_timings(backend, A, B, C, R, rows, k, timed)

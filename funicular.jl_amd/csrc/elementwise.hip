@@ -142,6 +142,85 @@ __global__ __launch_bounds__(256) void helmholtz_kernel(T *__restrict__ Y, int64
   }
 }
 
+// The same two stencils on a 1-D grid of (column, 256-row chunk) workgroups,
+// for N < 2^31 and fewer than 2^31 workgroups: 32-bit indices, the divisions
+// by n1, n2 and the chunk count as multiply-highs, and the workgroups of one
+// column renumbered so each XCD sweeps a contiguous range of it. The +-n1 and
+// +-n1 n2 neighbours of a row then sit in rows the same XCD read a few hundred
+// workgroups earlier, within its L2.
+template <typename T>
+__global__ __launch_bounds__(256) void tridiag_rows_kernel(T *__restrict__ Y, int64_t ldy, const T *__restrict__ X,
+                                                           int64_t ldx, uint32_t N, DivU32 chunks, T dr, T di, T orr,
+                                                           T oi, const double *__restrict__ diag) {
+  const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t col = divu32(b, chunks);
+  const uint32_t i = (b - col * chunks.d) * 256u + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t im1 = i == 0 ? N - 1 : i - 1;
+  const uint32_t ip1 = i == N - 1 ? 0 : i + 1;
+  const T *x = X + 2 * (int64_t)col * ldx;
+  T xr, xi, ar, ai, br, bi;
+  load_elem<T, T, true>(x, i, true, xr, xi);
+  load_elem<T, T, true>(x, im1, true, ar, ai);
+  load_elem<T, T, true>(x, ip1, true, br, bi);
+  T cr = dr, ci = di;
+  if (diag) cr += (T)diag[i];
+  const T nr = ar + br, ni = ai + bi;
+  const T yr = cr * xr - ci * xi + orr * nr - oi * ni;
+  const T yi = cr * xi + ci * xr + orr * ni + oi * nr;
+  store_elem<T, true>(Y + 2 * (int64_t)col * ldy, i, yr, yi);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void helmholtz_rows_kernel(T *__restrict__ Y, int64_t ldy,
+                                                             const T *__restrict__ X, int64_t ldx, uint32_t n1,
+                                                             uint32_t n2, uint32_t n3, DivU32 d1, DivU32 d2,
+                                                             DivU32 chunks, T dr, T di, T orr, T oi) {
+  const uint32_t s3 = n1 * n2, N = s3 * n3;
+  const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t col = divu32(b, chunks);
+  const uint32_t p = (b - col * chunks.d) * 256u + threadIdx.x;
+  if (p >= N) return;
+  const uint32_t jl = divu32(p, d1), i = p - jl * n1;
+  const uint32_t l = divu32(jl, d2), j = jl - l * n2;
+  const uint32_t im = i == 0 ? p + (n1 - 1) : p - 1;
+  const uint32_t ip = i == n1 - 1 ? p - (n1 - 1) : p + 1;
+  const uint32_t jm = j == 0 ? p + (n2 - 1) * n1 : p - n1;
+  const uint32_t jp = j == n2 - 1 ? p - (n2 - 1) * n1 : p + n1;
+  const uint32_t lm = l == 0 ? p + (n3 - 1) * s3 : p - s3;
+  const uint32_t lp = l == n3 - 1 ? p - (n3 - 1) * s3 : p + s3;
+  const T *x = X + 2 * (int64_t)col * ldx;
+  T xr, xi, sr, si, tr, ti;
+  load_elem<T, T, true>(x, p, true, xr, xi);
+  load_elem<T, T, true>(x, im, true, sr, si);
+  load_elem<T, T, true>(x, ip, true, tr, ti);
+  sr += tr;
+  si += ti;
+  load_elem<T, T, true>(x, jm, true, tr, ti);
+  sr += tr;
+  si += ti;
+  load_elem<T, T, true>(x, jp, true, tr, ti);
+  sr += tr;
+  si += ti;
+  load_elem<T, T, true>(x, lm, true, tr, ti);
+  sr += tr;
+  si += ti;
+  load_elem<T, T, true>(x, lp, true, tr, ti);
+  sr += tr;
+  si += ti;
+  const T yr = dr * xr - di * xi + orr * sr - oi * si;
+  const T yi = dr * xi + di * xr + orr * si + oi * sr;
+  store_elem<T, true>(Y + 2 * (int64_t)col * ldy, p, yr, yi);
+}
+
+// Workgroups for the row-chunk stencils, or 0 when the sizes need the 64-bit
+// grid-stride forms.
+static uint32_t rows_grid(int64_t N, int64_t w, int64_t &chunks) {
+  chunks = cdiv64(N, 256);
+  if (N >= (int64_t(1) << 31) || chunks * w >= (int64_t(1) << 31)) return 0;
+  return (uint32_t)(chunks * w);
+}
+
 template <typename T, bool C>
 __global__ __launch_bounds__(256) void axpby_kernel(T *__restrict__ Y, int64_t ldy, const T *__restrict__ X,
                                                     int64_t ldx, int64_t rows, int64_t cols, T ar, T ai, T br,
@@ -409,8 +488,20 @@ extern "C" int fhip_stencil_tridiag(void *Y, int64_t ldy, const void *X, int64_t
     d_im = -d_im;
     o_im = -o_im;
   }
-  const unsigned g = grid_for(N * w);
   hipStream_t s = (hipStream_t)stream;
+  int64_t chunks;
+  if (const uint32_t nb = rows_grid(N, w, chunks)) {
+    const DivU32 dc = make_divu32((uint32_t)chunks);
+    if (type == FHIP_C64)
+      hipLaunchKernelGGL((tridiag_rows_kernel<double>), dim3(nb), dim3(256), 0, s, (double *)Y, ldy,
+                         (const double *)X, ldx, (uint32_t)N, dc, d_re, d_im, o_re, o_im, diag);
+    else
+      hipLaunchKernelGGL((tridiag_rows_kernel<float>), dim3(nb), dim3(256), 0, s, (float *)Y, ldy,
+                         (const float *)X, ldx, (uint32_t)N, dc, (float)d_re, (float)d_im, (float)o_re,
+                         (float)o_im, diag);
+    return check_launch("tridiag_rows_kernel");
+  }
+  const unsigned g = grid_for(N * w);
   if (type == FHIP_C64)
     hipLaunchKernelGGL((tridiag_kernel<double>), dim3(g), dim3(256), 0, s, (double *)Y, ldy, (const double *)X,
                        ldx, N, w, d_re, d_im, o_re, o_im, diag);
@@ -436,8 +527,22 @@ extern "C" int fhip_stencil_helmholtz3d(void *Y, int64_t ldy, const void *X, int
     d_im = -d_im;
     o_im = -o_im;
   }
-  const unsigned g = grid_for(N * w);
   hipStream_t s = (hipStream_t)stream;
+  int64_t chunks;
+  if (const uint32_t nb = rows_grid(N, w, chunks)) {
+    const DivU32 d1 = make_divu32((uint32_t)n1), d2 = make_divu32((uint32_t)n2);
+    const DivU32 dc = make_divu32((uint32_t)chunks);
+    if (type == FHIP_C64)
+      hipLaunchKernelGGL((helmholtz_rows_kernel<double>), dim3(nb), dim3(256), 0, s, (double *)Y, ldy,
+                         (const double *)X, ldx, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, d1, d2, dc, d_re,
+                         d_im, o_re, o_im);
+    else
+      hipLaunchKernelGGL((helmholtz_rows_kernel<float>), dim3(nb), dim3(256), 0, s, (float *)Y, ldy,
+                         (const float *)X, ldx, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, d1, d2, dc,
+                         (float)d_re, (float)d_im, (float)o_re, (float)o_im);
+    return check_launch("helmholtz_rows_kernel");
+  }
+  const unsigned g = grid_for(N * w);
   if (type == FHIP_C64)
     hipLaunchKernelGGL((helmholtz_kernel<double>), dim3(g), dim3(256), 0, s, (double *)Y, ldy,
                        (const double *)X, ldx, n1, n2, n3, w, d_re, d_im, o_re, o_im);

@@ -20,6 +20,41 @@ inline bool valid_type(int type) { return type >= FHIP_F32 && type <= FHIP_C64; 
 
 inline int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Unsigned 32-bit division by a divisor fixed for a launch, as one
+// multiply-high, a subtract and two shifts (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, Fig. 4.1).
+// With l = ceil(log2 d): m = floor(2^32 (2^l - d) / d) + 1 (< 2^32), and for
+// every 32-bit n, n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0) with
+// t = mulhi(m, n).
+struct DivU32 {
+  uint32_t d, m, s1, s2;
+};
+
+inline DivU32 make_divu32(uint32_t d) {
+  uint32_t l = 0;
+  while (l < 32 && (uint64_t(1) << l) < d) ++l;
+  DivU32 r;
+  r.d = d;
+  r.m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+  r.s1 = l < 1 ? l : 1;
+  r.s2 = l > 1 ? l - 1 : 0;
+  return r;
+}
+
+__device__ inline uint32_t divu32(uint32_t n, const DivU32 &f) {
+  const uint32_t t = __umulhi(f.m, n);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// Hardware dispatches workgroup b of a 1-D grid to XCD b % 8. This renumbers
+// b so that each XCD takes one contiguous range of the nb logical ids (XCD x
+// takes nb / 8 of them, one more when x < nb % 8), so neighbouring work shares
+// that XCD's L2.
+__device__ inline uint32_t xcd_contiguous(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, local = b >> 3, per = nb >> 3, extra = nb & 7u;
+  return x * per + (x < extra ? x : extra) + local;
+}
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 

@@ -57,6 +57,13 @@ def assert_operator_shape(G, m, n):
                             f"columns of an {m}-row one, so it has to be {m}×{n}")
 
 
+def _stencil_eltypes(G, Y, X):
+    """The stencil kernels read and write the operator's own eltype."""
+    if X.dtype != G.dtype or Y.dtype != G.dtype:
+        raise ArgumentError(f"a {G.dtype} stencil maps {G.dtype} columns, got {X.dtype} onto {Y.dtype}. Build the "
+                            f"operator with dtype={X.dtype} to apply it to them")
+
+
 def apply_operator(dst, G, src):
     """src/operator.jl:80."""
     if panel_capable(G):
@@ -156,6 +163,7 @@ class Tridiagonal1D:
     def apply(self, Y, X):
         if X.rows != self.N or Y.rows != self.N or X.cols != Y.cols:
             raise ArgumentError(f"a {self.N}×{self.N} stencil cannot map {X.shape} onto {Y.shape}")
+        _stencil_eltypes(self, Y, X)
         call("fhip_stencil_tridiag", Y.ptr, Y.ld, X.ptr, X.ld, self.N, X.cols, self.d.real, self.d.imag,
              self.o.real, self.o.imag, None if self.diag is None else self.diag.ptr, 1 if self._adjoint else 0,
              eltype_code(self.dtype), current_stream())
@@ -206,6 +214,7 @@ class Helmholtz3D:
     def apply(self, Y, X):
         if X.rows != self.N or Y.rows != self.N or X.cols != Y.cols:
             raise ArgumentError(f"a {self.N}×{self.N} stencil cannot map {X.shape} onto {Y.shape}")
+        _stencil_eltypes(self, Y, X)
         n1, n2, n3 = self.n
         call("fhip_stencil_helmholtz3d", Y.ptr, Y.ld, X.ptr, X.ld, n1, n2, n3, X.cols, self.d.real, self.d.imag,
              self.o.real, self.o.imag, 1 if self._adjoint else 0, eltype_code(self.dtype), current_stream())

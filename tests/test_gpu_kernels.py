@@ -328,3 +328,50 @@ def test_stencils(backend):
     assert np.allclose(backend.to_host(Yd), OO.helmholtz_apply(X, n, OO.helmholtz_diagonal(), adjoint=True),
                        rtol=1e-14, atol=1e-14)
     assert F.check_operator(H, seed=1)
+
+
+@pytest.mark.parametrize("n,w", [((216, 216, 216), 2), ((37, 11, 5), 3), ((1, 1, 7), 2), ((5, 1, 1), 1)])
+def test_helmholtz_rows_at_size(backend, n, w):
+    """The row-chunk stencil at the benchmark's 216^3 grid (each division by n1
+    and n2 runs over every row index below 10,077,696), on odd grids, and on
+    grids with an edge of 1, in views with leading dimensions above N."""
+    import funicular_hip as F
+    rng = np.random.default_rng(20)
+    N = n[0] * n[1] * n[2]
+    X = randmatrix(rng, N, w, np.complex128)
+    pad = 0 if N > 10 ** 6 else 13
+    Xd = backend.todevice(np.asfortranarray(np.vstack([X, np.zeros((pad, w))])))
+    Yd = backend.alloc_device(np.complex128, (N + pad, w))
+    H = F.Helmholtz3D(*n, OO.helmholtz_diagonal(), -1.0, backend=backend)
+    for adj in (False, True):
+        Hop = H.adjoint() if adj else H
+        Hop.apply(Yd.view((0, N), None), Xd.view((0, N), None))
+        got = backend.to_host(Yd)[:N]
+        ref = OO.helmholtz_apply(X, n, OO.helmholtz_diagonal(), adjoint=adj)
+        assert np.allclose(got, ref, rtol=1e-14, atol=1e-14)
+    X32 = X.astype(np.complex64)
+    Xd32 = backend.todevice(np.asfortranarray(X32))
+    Yd32 = backend.alloc_device(np.complex64, (N, w))
+    with pytest.raises(F.ArgumentError):
+        H.apply(Yd32, Xd32)
+    F.Helmholtz3D(*n, OO.helmholtz_diagonal(), -1.0, backend=backend, dtype=np.complex64).apply(Yd32, Xd32)
+    ref = OO.helmholtz_apply(X32.astype(np.complex128), n, OO.helmholtz_diagonal())
+    assert np.allclose(backend.to_host(Yd32), ref, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("N", [1, 2, 257, 3 * 2 ** 20 + 5])
+def test_tridiag_rows_at_size(backend, N):
+    import funicular_hip as F
+    rng = np.random.default_rng(21)
+    w = 3
+    X = randmatrix(rng, N, w, np.complex128)
+    diag = 1 + 0.5 * np.sin(2 * np.pi * np.arange(N) / N)
+    Xd = backend.todevice(X)
+    Yd = backend.alloc_device(np.complex128, (N, w))
+    for dg in (None, diag):
+        G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, diag=dg, backend=backend)
+        G.apply(Yd, Xd)
+        assert np.allclose(backend.to_host(Yd), OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg), rtol=1e-14, atol=1e-14)
+        G.adjoint().apply(Yd, Xd)
+        assert np.allclose(backend.to_host(Yd), OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg, adjoint=True),
+                           rtol=1e-14, atol=1e-14)
