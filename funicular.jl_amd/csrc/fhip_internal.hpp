@@ -164,6 +164,21 @@ __device__ inline void mma_steps(const T *__restrict__ As_re, const T *__restric
 // 16-row x 2-column piece of a slice touches 32 different bank pairs.
 __device__ inline int lds_swizzle(int kr) { return ((kr & 1) << 4) | (((kr >> 1) & 7) << 1); }
 
+// The 4-byte form, for FP32 planes exactly 64 columns wide: column c of row kr
+// sits at c ^ lds_swizzle4(kr). A matrix-core operand read (lanes li + 16 lk
+// reading row 4 s + lk, columns 16-aligned + li) then puts its four rows in
+// four different 16-bank groups, and a staging store (lanes kk + 16 cc writing
+// row kk, column 4-aligned + cc, 16 rows per slice) touches 64 different banks;
+// both hold for 64 banks serving a whole wave and for 32 banks serving half a
+// wave. Padding instead (rows of 68) leaves the reads 4-way conflicted.
+__device__ inline int lds_swizzle4(int kr) { return ((kr & 3) << 4) | (kr & 2) | (((kr >> 2) & 3) << 2); }
+
+// Layout kinds: 0 plain (padded rows), 1 lds_swizzle, 2 lds_swizzle4.
+template <int SWZ>
+__device__ inline int swizzle_of(int kr) {
+  return SWZ == 1 ? lds_swizzle(kr) : (SWZ == 2 ? lds_swizzle4(kr) : 0);
+}
+
 // The same K steps for complex operands with three real products per complex
 // product instead of four (the 3M method; Higham, "Stability of a method for
 // multiplying complex matrices with three real matrix multiplications", SIMAX
@@ -174,7 +189,7 @@ __device__ inline int lds_swizzle(int kr) { return ((kr & 1) << 4) | (((kr >> 1)
 // count down from whatever the accumulators started at. The error bound is
 // normwise, |error| <= c u |a| |b| per product, the same order as the
 // four-product form's for these column-scaled sums.
-template <typename T, int MB, int NB, int MODE, bool SWZ = false>
+template <typename T, int MB, int NB, int MODE, int SWZ = 0>
 __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__restrict__ As_im, int lda_s,
                                     int mi0, const T *__restrict__ Bs_re, const T *__restrict__ Bs_im,
                                     int ldb_s, int nj0, int s_begin, int s_end, int s_stride,
@@ -185,7 +200,7 @@ __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__rest
   const int lk = lane >> 4;
   for (int s = s_begin; s < s_end; s += s_stride) {
     const int kr = 4 * s + lk;
-    const int sw = SWZ ? lds_swizzle(kr) : 0;
+    const int sw = swizzle_of<SWZ>(kr);
     T ar[MB], ai[MB], as[MB], br[NB], bi[NB], bt[NB];
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) {

@@ -135,10 +135,13 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
   constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
   constexpr int MB = GramShape<TILE>::MB;
   constexpr int NB = GramShape<TILE>::NB;
-  // Complex FP64 tiles of 32 or more columns use the swizzled LDS layout (no
-  // padding); the others pad each row by 4. The swizzle is laid out for 8-byte
-  // elements: with FP32 planes it measured slower than the padding.
-  constexpr bool SWZ = CPLX && sizeof(TC) == 8 && FHIP_GRAM_SWIZZLE && TILE >= 32;
+  // Complex tiles use a swizzled LDS layout with unpadded rows: the 8-byte
+  // swizzle for FP64 tiles of 32 or more columns, the 4-byte one for FP32
+  // tiles of 64. The others pad each row by 4.
+  constexpr int SWZ = !(CPLX && FHIP_GRAM_SWIZZLE) ? 0
+                      : (sizeof(TC) == 8 && TILE >= 32) ? 1
+                      : (sizeof(TC) == 4 && TILE == 64 && KC == 16) ? 2
+                                                                   : 0;
   constexpr int LDS_W = SWZ ? TILE : TILE + 4;
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int PER_THREAD = KC * TILE / GRAM_THREADS;
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
       const int idx = t + GRAM_THREADS * e;
       const int kk = idx % KC;
       const int c = idx / KC;
-      const int at = kk * LDS_W + (SWZ ? (c ^ lds_swizzle(kk)) : c);
+      const int at = kk * LDS_W + (c ^ swizzle_of<SWZ>(kk));
       if constexpr (CONVERTED) {
         TC re, im;
         from_raw(ra[e], oka[e], re, im);

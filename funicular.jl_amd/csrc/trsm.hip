@@ -234,9 +234,13 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
   constexpr int TW_KC = TrsmSlice<T>::KC;
   constexpr int TW_APT = TrsmSlice<T>::APT;
   constexpr int TW_BPT = TrsmSlice<T>::BPT;
-  // Complex FP64 planes use the swizzled LDS layout (laid out for 8-byte
-  // elements); the others pad each row by 4.
-  constexpr bool SWZ = CPLX && sizeof(T) == 8 && FHIP_TRSM_SWIZZLE;
+  // Complex planes use a swizzled LDS layout with unpadded rows: the 8-byte
+  // swizzle for FP64, the 4-byte one for FP32 (whose planes must be 64 wide
+  // with 16-row slices); the others pad each row by 4.
+  constexpr int SWZ = !(CPLX && FHIP_TRSM_SWIZZLE) ? 0
+                      : sizeof(T) == 8                                     ? 1
+                      : (TW_ROWS == 64 && TW_CB == 64 && TW_KC == 16)      ? 2
+                                                                           : 0;
   constexpr int TW_LDA = SWZ ? TW_ROWS : TW_ROWS + 4;
   constexpr int TW_LDB = SWZ ? TW_CB : TW_CB + 4;
   __shared__ T As[NP][TW_KC * TW_LDA];
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       const int idx = t + TW_THREADS * e;
       const int r = idx % TW_ROWS;
       const int kk = idx / TW_ROWS;
-      const int at = kk * TW_LDA + (SWZ ? (r ^ lds_swizzle(kk)) : r);
+      const int at = kk * TW_LDA + (r ^ swizzle_of<SWZ>(kk));
       if constexpr (CONVERTED) from_raw(ya[e], oka[e], ar[e], ai[e]);
       As[0][at] = ar[e];
       if (CPLX) As[NP - 1][at] = ai[e];
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       const int idx = t + TW_THREADS * e;
       const int kk = idx % TW_KC;
       const int c = idx / TW_KC;
-      const int at = kk * TW_LDB + (SWZ ? (c ^ lds_swizzle(kk)) : c);
+      const int at = kk * TW_LDB + (c ^ swizzle_of<SWZ>(kk));
       Bs[0][at] = bre[e];
       if (CPLX) Bs[NP - 1][at] = bim[e];
     }
