@@ -156,13 +156,16 @@ __device__ inline void mma_steps(const T *__restrict__ As_re, const T *__restric
   }
 }
 
-// LDS column swizzle for [k][column] operand planes whose rows are 16 or
-// more columns wide: column c of row kr sits at c ^ lds_swizzle(kr). It flips
-// bit 4 between even and odd rows, so the two rows a half wave reads for one
-// matrix-core operand land in different halves of the banks, and spreads the
-// 16 rows of a slice over the even offsets below 32, so a half wave storing a
-// 16-row x 2-column piece of a slice touches 32 different bank pairs.
-__device__ inline int lds_swizzle(int kr) { return ((kr & 1) << 4) | (((kr >> 1) & 7) << 1); }
+// LDS column swizzle for FP64 [k][column] operand planes whose rows are 32 or
+// more columns wide: column c of row kr sits at c ^ lds_swizzle(kr). Bit 4
+// flips between even and odd rows, so the two rows a half wave reads for one
+// matrix-core operand land in different halves of the banks; the low four
+// bits take the row's own low four bits, so the 16 rows of a slice that a
+// staging store writes (16 lanes, one column; or 32 lanes, two columns) fall
+// on distinct bank pairs whether 16 or 32 lanes of a 64-bit access are served
+// together. (An earlier form put only 8 distinct values in the low bits,
+// which left the 16-lane stores 2-way conflicted.)
+__device__ inline int lds_swizzle(int kr) { return (kr & 15) | ((kr & 1) << 4); }
 
 // The 4-byte form, for FP32 planes exactly 64 columns wide: column c of row kr
 // sits at c ^ lds_swizzle4(kr). A matrix-core operand read (lanes li + 16 lk
