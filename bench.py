@@ -602,14 +602,16 @@ def kernels_only(rows, k, reps, vendor=False):
     print(json.dumps(out), flush=True)
 
 
-def streamed(args):
+def streamed(args, phase_gap=0.0):
     """The reference's own regime: the basis lives in page-locked host memory
     (host_budget 128 GiB), every sweep streams row blocks up (and, for the
     solve, back down) through nbuffers = 2 staging buffers in a 64 GiB device
     budget, copies on their own streams fenced by events. Reports the
     PCIe-inclusive cholqr2 rate, the host-link GB/s, and the fraction of
     transfer hidden behind compute for a Gram sweep,
-    (t_copy + t_compute - t_wall) / min(t_copy, t_compute)."""
+    (t_copy + t_compute - t_wall) / min(t_copy, t_compute). phase_gap > 0
+    leaves the GPU idle that many seconds between the phases (panelmul, the
+    two copy-only sweeps, the Gram sweep, cholqr2, the final Gram)."""
     import funicular_hip as F
     from funicular_hip.executor import readpanels, rowtraversal, sweep
     backend = F.hip_backend()
@@ -635,6 +637,8 @@ def streamed(args):
 
     def timed(fn):
         backend.sync_all()
+        if phase_gap > 0:
+            time.sleep(phase_gap)
         c0 = F._lib.counters()
         t = time.perf_counter()
         out = fn()
@@ -656,6 +660,8 @@ def streamed(args):
         f"gram sweep {t_gram:.2f} s with {t_compute:.3f} s of kernels; hidden {hidden:.3f}")
     t_chol, up, down, R = timed(lambda: F.cholqr2(Y))
     log(f"streamed: cholqr2 {t_chol:.2f} s, {N / t_chol:.0f} rows/s, up {up / 1e9:.1f} GB, down {down / 1e9:.1f} GB")
+    if phase_gap > 0:
+        time.sleep(phase_gap)
     orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
     return {"cholqr2_s": t_chol, "rows_per_s_pcie_inclusive": N / t_chol, "h2d_GB": up / 1e9,
             "d2h_GB": down / 1e9, "link_GBps_during_cholqr2": (up + down) / t_chol / 1e9,
@@ -724,6 +730,9 @@ def main():
     ap.add_argument("--stream-grid", type=int, default=180, help="grid edge of the streamed run")
     ap.add_argument("--streamed-narrow", action="store_true",
                     help="also run the full-size streamed cholqr2 with ComplexF32 host storage")
+    ap.add_argument("--streamed-only", action="store_true",
+                    help="profile mode: only the streamed run, its phases separated by 0.25 s of GPU idle so "
+                         "a trace can be cut at the gaps")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profile mode: only the per-launch kernels at the row-block shape")
     ap.add_argument("--vendor", action="store_true",
@@ -731,6 +740,9 @@ def main():
     args = ap.parse_args()
     if args.kernels_only:
         kernels_only(args.grid ** 3 * args.w // args.k, args.k, max(args.steps, 1), vendor=args.vendor)
+        return
+    if args.streamed_only:
+        print(json.dumps(streamed(args, phase_gap=0.25)), flush=True)
         return
 
     if args.gpus != 1:

@@ -375,3 +375,29 @@ def test_tridiag_rows_at_size(backend, N):
         G.adjoint().apply(Yd, Xd)
         assert np.allclose(backend.to_host(Yd), OO.tridiag_apply(X, 2 + 0.1j, -1.0, dg, adjoint=True),
                            rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("m,k", [(257, 33), (1000, 64), (3001, 100), (2048, 200)])
+@pytest.mark.parametrize("stored", [np.complex64, np.complex128])
+def test_fp32_products_over_tile_widths(backend, m, k, stored):
+    """The FP32 Gram and solve on their swizzled 64-wide LDS planes, at widths
+    that leave a ragged last tile and column block: native ComplexF32 data and
+    FP32 products of ComplexF64 data."""
+    rng = np.random.default_rng(m + k)
+    A = randmatrix(rng, m, k, stored)
+    Ad = backend.todevice(A)
+    C = backend.alloc_device(np.complex128 if stored == np.complex128 else np.complex64, (k, k))
+    backend.gram_accumulate(C, Ad, Ad, 1.0, 0.0, compute=np.complex64)
+    H = backend.to_host(C)
+    A64 = A.astype(np.complex128)
+    ref = A64.conj().T @ A64
+    assert np.array_equal(H, H.conj().T)
+    assert np.linalg.norm(H - ref) <= 1e-5 * np.linalg.norm(ref)
+    # The solve by R of the FP64 Gram, in FP32 products.
+    R = np.asfortranarray(np.linalg.cholesky(ref).conj().T)
+    Yd = backend.todevice(A)
+    backend.rdiv_upper(Yd, backend.todevice(R.astype(stored)), compute=np.complex64)
+    X = backend.to_host(Yd).astype(np.complex128)
+    Xr = OB.rdiv_upper(A64, R)
+    assert np.linalg.norm(X - Xr) <= 1e-4 * np.linalg.norm(Xr)
+    assert np.linalg.norm(X.conj().T @ X - np.eye(k)) <= 1e-3

@@ -369,3 +369,38 @@ def test_narrowed_host_storage(backend):
     X = F.PanelMatrix.from_array(A, plan=plan, w=w)
     stored = A.astype(np.complex64).astype(C16)
     assert OB.isapprox(F.gram(X), stored.conj().T @ stored, OB.blastol(C16, N, k))
+
+
+@pytest.mark.parametrize("nb", (1, 2, 3))
+@pytest.mark.parametrize("narrow", (False, True))
+def test_row_sweep_with_read_and_updated_operands(backend, nb, narrow):
+    """A row-block sweep that reads one host matrix and updates another: each
+    upload into a slot waits for the compute that last read it (read-only
+    operand) or, panel by panel, for the write-back that last read its columns
+    (updated operand). Every step's result must land exactly, at every
+    nbuffers, with the updated blocks narrowed to ComplexF32 on the host too."""
+    rng = np.random.default_rng(900 + nb)
+    N, k, w = 3000, 96, 4
+    A, B = randmatrix(rng, N, k), randmatrix(rng, N, k)
+    kw = {"host_eltype": np.complex64} if narrow else {}
+    plan = make_plan(backend, **kw)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Y = F.PanelMatrix.from_array(B, plan=plan, w=w)
+    t = rowtraversal(Y)
+    assert len(t.blocks) > 3 * nb
+
+    def f(_, xb, yb):
+        backend.axpby(yb, xb, 0.5, 2.0)  # yb <- 0.5 xb + 2 yb
+
+    sweep(f, readpanels(X), updatepanels(Y), nbuffers=nb, traversal=t)
+    sweep(f, readpanels(X), updatepanels(Y), nbuffers=nb, traversal=t)
+    if narrow:
+        A, B = (M.astype(np.complex64).astype(C16) for M in (A, B))
+        once = (0.5 * A + 2 * B).astype(np.complex64).astype(C16)
+        expect = (0.5 * A + 2 * once).astype(np.complex64).astype(C16)
+        assert np.allclose(F.to_matrix(Y), expect, rtol=0, atol=1e-6)
+    else:
+        expect = 0.5 * A + 2 * (0.5 * A + 2 * B)
+        assert np.allclose(F.to_matrix(Y), expect, rtol=1e-15, atol=1e-15)
+    assert np.array_equal(F.to_matrix(X), A)
+    assert all(p.pins == 0 for p in X.panels + Y.panels)
