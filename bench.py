@@ -611,7 +611,8 @@ def streamed(args, phase_gap=0.0):
     transfer hidden behind compute for a Gram sweep,
     (t_copy + t_compute - t_wall) / min(t_copy, t_compute). phase_gap > 0
     leaves the GPU idle that many seconds between the phases (panelmul, the
-    two copy-only sweeps, the Gram sweep, cholqr2, the final Gram)."""
+    two copy-only sweeps, the Gram sweep, cholqr2, project by runs, project
+    on the reference schedule, the final Gram)."""
     import funicular_hip as F
     from funicular_hip.executor import readpanels, rowtraversal, sweep
     backend = F.hip_backend()
@@ -660,6 +661,13 @@ def streamed(args, phase_gap=0.0):
         f"gram sweep {t_gram:.2f} s with {t_compute:.3f} s of kernels; hidden {hidden:.3f}")
     t_chol, up, down, R = timed(lambda: F.cholqr2(Y))
     log(f"streamed: cholqr2 {t_chol:.2f} s, {N / t_chol:.0f} rows/s, up {up / 1e9:.1f} GB, down {down / 1e9:.1f} GB")
+    # project on the orthonormal basis: by runs of held panel images (the
+    # default off HBM), then the reference's p (p + 1)-step schedule.
+    t_proj, up_proj, _, S = timed(lambda: F.project(Y, G))
+    t_ref, up_ref, _, S_ref = timed(lambda: F.project(Y, G, group=1))
+    agree = float(np.linalg.norm(S - S_ref) / np.linalg.norm(S_ref))
+    log(f"streamed: project {t_proj:.2f} s ({up_proj / 1e9:.0f} GB up) against {t_ref:.2f} s "
+        f"({up_ref / 1e9:.0f} GB up) on the reference schedule; ||S - S_ref|| / ||S_ref|| = {agree:.1e}")
     if phase_gap > 0:
         time.sleep(phase_gap)
     orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
@@ -668,6 +676,9 @@ def streamed(args, phase_gap=0.0):
             "copy_only_sweep_nb1_s": t_copy1, "copy_only_sweep_nb2_s": t_copy2,
             "h2d_GBps_copy_only": up1 / t_copy1 / 1e9, "gram_sweep_s": t_gram, "gram_kernels_s": t_compute,
             "hidden_fraction": hidden, "panelmul_s": t_panelmul, "orthogonality": orth,
+            "project_s": t_proj, "project_h2d_GB": up_proj / 1e9, "project_reference_schedule_s": t_ref,
+            "project_reference_schedule_h2d_GB": up_ref / 1e9, "project_relative_difference": agree,
+            "project_check": project_check(S, d),
             "N": N, "budgets": "device 64 GiB, host 60 GiB page-locked, nbuffers 2"}
 
 

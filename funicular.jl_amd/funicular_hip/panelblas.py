@@ -405,20 +405,31 @@ def _pool_can_hold(plan, T, dims):
     return pool.allocated + int(dims[0]) * int(dims[1]) * np.dtype(T).itemsize <= pool.budget
 
 
-def project(Q, G, *, nbuffers=None):
+def project(Q, G, *, nbuffers=None, group=None):
     """S = Q^H G Q without storing G Q (src/panelblas.jl:247-301).
 
     A basis resident in HBM whose plan can lend a second N x k buffer takes a
     shorter route, since no panel has to travel: G Q is formed whole in that
     pooled buffer, one stencil pass per panel, and a single Gram sweep gives
     all of S on 64 x 64 matrix-core tiles, instead of p^2 reductions of 32 x 32
-    tiles, each reading two whole panels from HBM."""
+    tiles, each reading two whole panels from HBM.
+
+    Otherwise the images of a run of g consecutive panels are held on the
+    device (g = `group`, by default as many as the plan's device pool can lend
+    beside one sweep's staging buffers): the run's panels are staged once to
+    form G Q_run, then every panel Q_i streams past it once for the blocks
+    Q_i^H (G Q_run) of S. That is p + p ceil(p / g) panel transfers instead of
+    the reference schedule's p (p + 1); with g = 1 the reference schedule runs
+    as it is, its labels included."""
     plan = Q.plan
     assert_square_operator(G, Q.N)
-    if Q.resident and _pool_can_hold(plan, Q.dtype, (Q.N, Q.k)):
+    if group is None and Q.resident and _pool_can_hold(plan, Q.dtype, (Q.N, Q.k)):
         with pooled_like(Q) as GQ:
             panelmul(GQ, G, Q, nbuffers=nbuffers)
             return gram(Q, GQ, nbuffers=nbuffers)
+    g = project_group(Q, nbuffers) if group is None else max(1, int(group))
+    if g > 1:
+        return _project_runs(Q, G, g, nbuffers)
     p = npanels(Q)
     T = Q.dtype
     backend = plan.backend
@@ -440,6 +451,64 @@ def project(Q, G, *, nbuffers=None):
                 block = accumulator.view((ri.start, ri.stop), (rj.start, rj.stop))
                 backend.gram_accumulate(block, Qb, GQj, 1.0, 0.0, work)
         sweep(step, readpanels(Q), nbuffers=nbuffers, traversal=PanelSteps(panels, labels))
+        return collect_small(accumulator, plan)
+    finally:
+        _plan.checkin_device_buffers(plan, [image, accumulator, work])
+
+
+def _project_runs_bytes(Q, g):
+    """Device bytes of project's run route with runs of g panels: the run's
+    images, the k x k accumulator and the largest Gram workspace."""
+    item = np.dtype(Q.dtype).itemsize
+    spans = sorted({c1 - c0 for _, _, c0, c1 in _groups(Q, g)})
+    widths = sorted({panelwidth(Q, i) for i in range(npanels(Q))})
+    backend = Q.plan.backend
+    work = max(backend.gram_workspace_bytes(Q.N, wi, ws, Q.dtype, hermitian=False) for wi in widths for ws in spans)
+    return Q.N * spans[-1] * item + Q.k * Q.k * item + work
+
+
+def project_group(Q, nbuffers=None):
+    """How many panel images project holds on the device at once: the longest
+    run whose images, accumulator and Gram workspace the plan's device pool can
+    still lend beside one sweep's staging buffers (with room for narrowed
+    staging too), at most every panel, at least one."""
+    plan = Q.plan
+    pool = plan.devicepool
+    panel = Q.N * Q.w * np.dtype(Q.dtype).itemsize
+    nb = plan.nbuffers if nbuffers is None else int(nbuffers)
+    spare = pool.budget - pool.allocated - (0 if Q.resident else 2 * nb * panel)
+    g = min(npanels(Q), max(1, spare // max(panel, 1)))
+    while g > 1 and _project_runs_bytes(Q, g) > spare:
+        g -= 1
+    return int(g)
+
+
+def _project_runs(Q, G, g, nbuffers):
+    """project with the images of g consecutive panels held on the device."""
+    plan = Q.plan
+    backend = plan.backend
+    T = Q.dtype
+    p = npanels(Q)
+    runs = list(_groups(Q, g))
+    widths = sorted({panelwidth(Q, i) for i in range(p)})
+    spans = sorted({c1 - c0 for _, _, c0, c1 in runs})
+    image = _plan.checkout_device_buffers(plan, T, (Q.N, spans[-1]), 1)[0]
+    accumulator = work = None
+    try:
+        accumulator = _plan.checkout_device_buffers(plan, T, (Q.k, Q.k), 1)[0]
+        need = max(backend.gram_workspace_bytes(Q.N, wi, ws, T, hermitian=False) for wi in widths for ws in spans)
+        work = _plan.checkout_workspace(plan, need)
+        for j0, j1, c0, c1 in runs:
+            def apply(j, Qb, c0=c0):
+                r = panelrange(Q, j)
+                apply_operator(image.view(None, (r.start - c0, r.stop - c0)), G, Qb)
+            sweep(apply, readpanels(Q), nbuffers=nbuffers, traversal=PanelSteps(list(range(j0, j1))))
+            GQrun = image.view(None, (0, c1 - c0))
+
+            def reduce(i, Qb, c0=c0, c1=c1, GQrun=GQrun):
+                ri = panelrange(Q, i)
+                backend.gram_accumulate(accumulator.view((ri.start, ri.stop), (c0, c1)), Qb, GQrun, 1.0, 0.0, work)
+            sweep(reduce, readpanels(Q), nbuffers=nbuffers, traversal=PanelSteps(list(range(p))))
         return collect_small(accumulator, plan)
     finally:
         _plan.checkin_device_buffers(plan, [image, accumulator, work])

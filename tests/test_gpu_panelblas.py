@@ -404,3 +404,50 @@ def test_row_sweep_with_read_and_updated_operands(backend, nb, narrow):
         assert np.allclose(F.to_matrix(Y), expect, rtol=1e-15, atol=1e-15)
     assert np.array_equal(F.to_matrix(X), A)
     assert all(p.pins == 0 for p in X.panels + Y.panels)
+
+
+@pytest.mark.parametrize("group", (1, 2, 3, None))
+def test_project_by_runs_of_held_images(backend, group):
+    """Off HBM, project holds the images of a run of panels on the device and
+    streams every panel past each run once: (1 + number of runs) uploads of the
+    basis, against p + 1 for the reference schedule (group 1, which it runs
+    unchanged). S agrees with the oracle at every run length, ragged last
+    panel and run included."""
+    rng = np.random.default_rng(950 + (group or 0))
+    N, k, w = 3000, 23, 4
+    A = randmatrix(rng, N, k)
+    d, o = 2 + 0.1j, -1.0
+    ref = OB.project(A, lambda X: OO.tridiag_apply(X, d, o), w)
+    plan = make_plan(backend)
+    G = F.Tridiagonal1D(N, d, o, backend=backend)
+    Q = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    p = F.npanels(Q)
+    g = F.panelblas.project_group(Q) if group is None else group
+    assert g == p or group is not None
+    runs = -(-p // g)
+    c0 = F._lib.counters()
+    S = F.project(Q, G, group=group)
+    c1 = F._lib.counters()
+    assert OB.isapprox(S, ref, OB.blastol(C16, N, k))
+    assert c1["h2d_bytes"] - c0["h2d_bytes"] == (1 + runs) * N * k * 16
+    assert np.array_equal(F.to_matrix(Q), A)
+    assert all(pn.pins == 0 for pn in Q.panels)
+
+
+def test_project_by_runs_over_a_disk_backed_basis(backend, tmp_path):
+    rng = np.random.default_rng(960)
+    N, k, w = 2000, 24, 4
+    A = randmatrix(rng, N, k)
+    d, o = 2 + 0.1j, -1.0
+    ref = OB.project(A, lambda X: OO.tridiag_apply(X, d, o), w)
+    slot = P.align_up(P.panel_bytes(N, w, np.dtype(C16)), P.SLAB_ALIGN)
+    plan = F.ResidencyPlan(backend=backend, device_budget=64 << 20, host_budget=4 * slot,
+                           scratch_dir=str(tmp_path), panel_width=w)
+    G = F.Tridiagonal1D(N, d, o, backend=backend)
+    Q = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    before = F.disk_reads(Q)
+    for group in (2, None):
+        S = F.project(Q, G, group=group)
+        assert OB.isapprox(S, ref, OB.blastol(C16, N, k))
+    assert F.disk_reads(Q) > before
+    assert np.array_equal(F.to_matrix(Q), A)
