@@ -688,7 +688,8 @@ def streamed_narrow(args):
     ComplexF32 (41.3 GB, within the 60 GiB budget), widened to ComplexF64 on
     the device as each row block arrives and narrowed on the way back, so the
     FP64 cholqr2 moves half the link bytes. Reports the PCIe-inclusive rate
-    and ||Q^H Q - I||_F of the stored (ComplexF32) Q."""
+    and ||Q^H Q - I||_F of the stored (ComplexF32) Q, then times project on
+    that Q (runs of held images)."""
     import funicular_hip as F
     backend = F.hip_backend()
     n = args.grid
@@ -705,6 +706,15 @@ def streamed_narrow(args):
     F.panelmul(Y, G, omega)
     backend.sync_all()
     t_panelmul = time.perf_counter() - t
+    # The link's own rate in this process: a copy-only row sweep (nbuffers 1).
+    from funicular_hip.executor import readpanels, rowtraversal, sweep
+    c0 = F._lib.counters()
+    t = time.perf_counter()
+    sweep(lambda *a: None, readpanels(Y), nbuffers=1, traversal=rowtraversal(Y))
+    backend.sync_all()
+    t_copy = time.perf_counter() - t
+    h2d_copy = (F._lib.counters()["h2d_bytes"] - c0["h2d_bytes"]) / t_copy / 1e9
+    log(f"streamed-narrow: copy-only sweep {t_copy:.2f} s, {h2d_copy:.1f} GB/s up")
     c0 = F._lib.counters()
     t = time.perf_counter()
     F.cholqr2(Y)
@@ -715,9 +725,21 @@ def streamed_narrow(args):
     orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
     log(f"streamed-narrow: cholqr2 {t_chol:.2f} s, {N / t_chol:.0f} rows/s, up {up / 1e9:.1f} GB, "
         f"down {down / 1e9:.1f} GB, ||Q^H Q - I||_F of the stored Q {orth:.2e}")
+    backend.sync_all()
+    g = F.panelblas.project_group(Y)
+    c0 = F._lib.counters()
+    t = time.perf_counter()
+    S = F.project(Y, G)
+    backend.sync_all()
+    t_proj = time.perf_counter() - t
+    up_proj = F._lib.counters()["h2d_bytes"] - c0["h2d_bytes"]
+    log(f"streamed-narrow: project {t_proj:.2f} s, {up_proj / 1e9:.0f} GB up, runs of {g} held images")
     return {"cholqr2_s": t_chol, "rows_per_s_pcie_inclusive": N / t_chol, "h2d_GB": up / 1e9,
             "d2h_GB": down / 1e9, "link_GBps_during_cholqr2": (up + down) / t_chol / 1e9,
-            "panelmul_s": t_panelmul, "stored_q_orthogonality": orth, "N": N,
+            "panelmul_s": t_panelmul, "h2d_GBps_copy_only": h2d_copy, "stored_q_orthogonality": orth,
+            "project_s": t_proj,
+            "project_h2d_GB": up_proj / 1e9, "project_run_length": g, "project_check": project_check(S, d),
+            "N": N,
             "budgets": "device 64 GiB, host 60 GiB page-locked, host_eltype ComplexF32, nbuffers 2"}
 
 

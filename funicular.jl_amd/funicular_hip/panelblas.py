@@ -15,8 +15,8 @@ from . import plan as _plan
 from .device_api import ArgumentError
 from .executor import PanelSteps, RowSteps, readpanels, rowtraversal, sweep, updatepanels, writepanels
 from .operator import apply_operator, assert_operator_shape, assert_square_operator
-from .panels import (assert_conformal, assert_same_panels, assert_same_rows, npanels, panelrange, panelwidth,
-                     pooled_like)
+from .panels import (assert_conformal, assert_same_panels, assert_same_rows, host_storage_eltype, npanels, panelrange,
+                     panelwidth, pooled_like)
 
 
 def shared_plan(X, Y):
@@ -470,13 +470,12 @@ def _project_runs_bytes(Q, g):
 def project_group(Q, nbuffers=None):
     """How many panel images project holds on the device at once: the longest
     run whose images, accumulator and Gram workspace the plan's device pool can
-    still lend beside one sweep's staging buffers (with room for narrowed
-    staging too), at most every panel, at least one."""
+    still lend beside one sweep's staging buffers (those it does not hold
+    already), at most every panel, at least one."""
     plan = Q.plan
     pool = plan.devicepool
     panel = Q.N * Q.w * np.dtype(Q.dtype).itemsize
-    nb = plan.nbuffers if nbuffers is None else int(nbuffers)
-    spare = pool.budget - pool.allocated - (0 if Q.resident else 2 * nb * panel)
+    spare = pool.budget - pool.allocated - _panel_staging_missing(Q, nbuffers)
     g = min(npanels(Q), max(1, spare // max(panel, 1)))
     while g > 1 and _project_runs_bytes(Q, g) > spare:
         g -= 1
@@ -520,15 +519,35 @@ def _project_runs(Q, G, g, nbuffers):
 # the device while the others stream past it, so only the run and the
 # staging buffers need to fit. The reference has no such traversal.
 
+def _panel_staging_missing(X, nbuffers):
+    """Bytes of the staging buffers a column-panel sweep over X needs (nb
+    panels, and nb more in the host storage eltype for a narrowed host tier)
+    that the plan's device pool does not already hold idle."""
+    if X.resident:
+        return 0
+    plan = X.plan
+    nb = plan.nbuffers if nbuffers is None else int(nbuffers)
+    dims = (int(X.N), int(X.w))
+    kinds = [np.dtype(X.dtype)]
+    S = np.dtype(host_storage_eltype(plan, X.dtype))
+    if S != kinds[0]:
+        kinds.append(S)
+    missing = 0
+    for T in kinds:
+        idle = len(plan.devicepool.available.get((T, dims), ()))
+        missing += max(0, nb - idle) * dims[0] * dims[1] * T.itemsize
+    return missing
+
+
 def panel_group(X, nbuffers=None):
     """How many consecutive column panels the panel-wise sweeps hold on the
     device at once: as many as the plan's device pool can still lend beside
-    one sweep's staging buffers, and at least one."""
+    one sweep's staging buffers (those it does not hold already) and the
+    kernels' workspaces, and at least one."""
     plan = X.plan
     pool = plan.devicepool
     panel = X.N * X.w * np.dtype(X.dtype).itemsize
-    nb = plan.nbuffers if nbuffers is None else int(nbuffers)
-    spare = pool.budget - pool.allocated - (nb + 2) * panel - (64 << 20)
+    spare = pool.budget - pool.allocated - _panel_staging_missing(X, nbuffers) - 2 * panel - (64 << 20)
     return int(max(1, min(npanels(X), spare // max(panel, 1))))
 
 
