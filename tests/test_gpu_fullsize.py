@@ -78,3 +78,40 @@ def test_mixed_cholqr2_meets_the_orthogonality_bar(backend, basis):
         # The FP32 solve leaves the factorization at FP32 level; the FP64
         # solve keeps it at FP64 level.
         assert res <= (1e-5 if solve == "fp32" else 1e-13), (solve, res)
+
+
+def test_sketch_chain_at_its_configured_size(backend):
+    """The 1-D sketch chain at its configured size, N = 2^20, k = 64 in panels
+    of 8: Omega = GhostPanels(seed 0xdeadbeef), Y = G Omega for the periodic
+    tridiagonal (2 + 0.1i, -1), cholqr2(Y), S = project(Q, G), against the
+    oracle's chain on the same Omega (the ghost stream itself is bit-pinned to
+    the oracle at smaller sizes). Bars: ||Q^H Q - I||_F <= 1e-13,
+    ||Y - Q R||_F / ||Y||_F <= 1e-13, S within 1e-12 normwise and entrywise
+    wherever |S_ij| >= 1e-3 max |S|."""
+    from oracle import operators as OO
+    from oracle import panelblas as OB
+    print("full size: the 2^20 x 64 sketch chain", flush=True)
+    N, k, w = 2 ** 20, 64, 8
+    d, o = 2 + 0.1j, -1.0
+    plan = F.ResidencyPlan(backend=backend, device_budget=16 * 1024 ** 3, host_budget=8 * 1024 ** 3)
+    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=SEED, w=w)
+    G = F.Tridiagonal1D(N, d, o, backend=backend)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    F.panelmul(Y, G, omega)
+    Om = F.to_matrix(omega)
+    Y0 = F.to_matrix(Y)
+    R = F.cholqr2(Y)
+    S = F.project(Y, G)
+    Q = F.to_matrix(Y)
+    apply = lambda X: OO.tridiag_apply(X, d, o)
+    Yo = OB.panelmul(apply, Om, w)
+    assert OB.isapprox(Y0, Yo, 1e-14)
+    Qo, Ro = OB.cholqr2(Yo, w)
+    So = OB.project(Qo, apply, w)
+    assert OB.orthogonality_error(Q) <= 1e-13
+    assert np.linalg.norm(Q @ R - Yo) / np.linalg.norm(Yo) <= 1e-13
+    assert OB.isapprox(R, Ro, 1e-12)
+    assert np.linalg.norm(S - So) <= 1e-12 * np.linalg.norm(So)
+    big = np.abs(So) >= 1e-3 * np.abs(So).max()
+    assert np.all(np.abs(S - So)[big] <= 1e-12 * np.abs(So)[big])
+    F.free(Y)
