@@ -81,3 +81,54 @@ def test_fp32_cholesky_shift_policy(backend):
     assert s > 0.0
     with pytest.raises(F.ArgumentError):
         cholqr2_mixed(F.PanelMatrix.from_array(np.asfortranarray(A), plan=make_plan(backend), w=5), solve="fp16")
+
+
+def test_mixed_workload_at_its_configured_size(backend):
+    """The mixed-precision workload at its configured size: Y = U diag(sigma)
+    V^H, 8,388,608 x 256 in panels of 16, resident in HBM, U the Q of cholqr2
+    of a ghost basis (seed 0xdeadbeef), V a seeded 256 x 256 unitary, sigma
+    log-spaced to 1e-2 and to 1e-4. The mixed first pass (FP32 Gram,
+    Cholesky and solve, or the solve in FP64) is held to the FP64-only run:
+    ||Q^H Q - I||_F <= 1e-13 for all three, ||Y - Q R|| / ||Y|| <= 1e-13 with
+    the FP64 solves and <= 1e-7 kappa with the FP32 one, and R within the
+    conditioning of the FP64-only R."""
+    from funicular_hip.executor import readpanels, sweep, updatepanels, writepanels
+    print("full size: the 8388608 x 256 mixed workload", flush=True)
+    N, k, w = 8388608, 256, 16
+    plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=1 << 30,
+                           device_resident=True)
+    U = F.PanelMatrix(N, k, plan=plan, w=w)
+    src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(U))
+    F.cholqr2(U)
+    rng = np.random.default_rng(0xDEADBEEF + 1)
+    V = np.linalg.qr(rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k)))[0]
+    Y0, Y, Z = (F.PanelMatrix(N, k, plan=plan, w=w) for _ in range(3))
+
+    def resid(R):
+        F.rightmul(Z, Y, R)
+        sweep(lambda _, a, b: backend.axpby(b, a, -1.0, 1.0), readpanels(Y0), updatepanels(Z))
+        return float(np.sqrt(np.real(np.trace(F.gram(Z))) / np.real(np.trace(F.gram(Y0)))))
+
+    try:
+        for decades in (2, 4):
+            kappa = 10.0 ** decades
+            sigma = 10.0 ** (-decades * np.arange(k) / (k - 1))
+            F.rightmul(Y0, U, sigma[:, None] * V.conj().T)
+            results = {}
+            for name, fn in (("fp64", F.cholqr2), ("mixed", cholqr2_mixed),
+                             ("mixed_fp64_solve", lambda X: cholqr2_mixed(X, solve="fp64"))):
+                sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(Y0), writepanels(Y))
+                R = fn(Y)
+                orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+                results[name] = (R, orth, resid(R))
+            R64 = results["fp64"][0]
+            for name, (R, orth, res) in results.items():
+                print(f"sigma to 1e-{decades} {name}: orthogonality {orth:.2e}, residual {res:.2e}", flush=True)
+                assert orth <= 1e-13, (name, orth)
+                assert res <= (1e-7 * kappa if name == "mixed" else 1e-13), (name, res)
+                tol = (1e-6 if name == "mixed" else 1e-14) * kappa
+                assert np.linalg.norm(R - R64) <= tol * np.linalg.norm(R64), name
+    finally:
+        for pm in (U, Y0, Y, Z):
+            F.free(pm)
