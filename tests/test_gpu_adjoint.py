@@ -171,3 +171,67 @@ def test_project_pullback_against_oracle(backend, w, resident):
     e = rng.standard_normal(N)
     fd = (loss(Qh, diag + h * e) - loss(Qh, diag - h * e)) / (2 * h)
     assert abs(fd - dbar @ e) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_adjoint_workload_at_its_configured_size(backend):
+    """The adjoint workload at its configured size: Y a 4,194,304 x 128 ghost
+    basis in panels of 32, resident in HBM; Q R = Y and S = Q^H G Q with
+    G = diag(d) + the periodic tridiagonal (2 + 0.1i, -1), d_i = 1 + 0.5
+    sin(2 pi i / N); loss Re<dQ, Q> + Re<dS, S>. Both pullbacks are checked
+    against central differences of the device forward pass, along a ghost
+    direction E of Y and a random direction e of d."""
+    from funicular_hip.executor import readpanels, sweep, updatepanels, writepanels
+    print("full size: the 4194304 x 128 adjoint workload", flush=True)
+    N, k, w = 1 << 22, 128, 32
+    plan = F.ResidencyPlan(backend=backend, device_budget=100 * 1024 ** 3, host_budget=1 << 30,
+                           device_resident=True)
+    d = 1.0 + 0.5 * np.sin(2 * np.pi * np.arange(N) / N)
+    G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, d, backend=backend)
+
+    def ghost_copy(seed):
+        src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=seed, w=w)
+        dst = F.PanelMatrix(N, k, plan=plan, w=w)
+        sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(dst))
+        return dst
+
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((k, k)) + 1j * rng.standard_normal((k, k))
+    dS = (A + A.conj().T) / 2
+    dQ = ghost_copy(0xDEADBEEF + 4)
+
+    def loss(Y, S):
+        return float(np.real(F.gram(dQ, Y).trace()) + np.real(np.vdot(dS, S)))
+
+    Y = ghost_copy(0xDEADBEEF + 3)
+    R, pb_qr = F.rrule_cholqr2(Y)
+    S, pb_pr = F.rrule_project(Y, G)
+    Qbar, dbar = pb_pr(dS)
+    sweep(lambda _, a, b: backend.axpby(b, a, 1.0, 1.0), readpanels(dQ), updatepanels(Qbar))
+    Ybar = pb_qr(Qbar, None)
+    E = ghost_copy(0xDEADBEEF + 5)
+    h = 1e-6
+
+    def shifted(sign):
+        Yh = ghost_copy(0xDEADBEEF + 3)
+        sweep(lambda _, e, y: backend.axpby(y, e, sign * h, 1.0), readpanels(E), updatepanels(Yh))
+        F.cholqr2(Yh)
+        out = loss(Yh, F.project(Yh, G))
+        F.free(Yh)
+        return out
+
+    fd = (shifted(1.0) - shifted(-1.0)) / (2 * h)
+    directional = float(np.real(F.gram(Ybar, E).trace()))
+    rel = abs(fd - directional) / abs(fd)
+    e = np.random.default_rng(5).standard_normal(N)
+
+    def shifted_d(sign):
+        Gh = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, d + sign * h * e, backend=backend)
+        return float(np.real(np.vdot(dS, F.project(Y, Gh))))
+
+    fd_d = (shifted_d(1.0) - shifted_d(-1.0)) / (2 * h)
+    rel_d = abs(fd_d - float(dbar @ e)) / abs(fd_d)
+    print(f"adjoint at full size: <Ybar, E> rel {rel:.2e}; dS/dd along e rel {rel_d:.2e}", flush=True)
+    assert rel <= 1e-6, rel
+    assert rel_d <= 1e-5, rel_d
+    for pm in (Y, dQ, Qbar, Ybar, E):
+        F.free(pm)
