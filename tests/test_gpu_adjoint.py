@@ -235,3 +235,6 @@ def test_adjoint_workload_at_its_configured_size(backend):
     assert rel_d <= 1e-5, rel_d
     for pm in (Y, dQ, Qbar, Ybar, E):
         F.free(pm)
+    del R, S, pb_qr, pb_pr, plan
+    import gc
+    gc.collect()

@@ -115,3 +115,5 @@ def test_sketch_chain_at_its_configured_size(backend):
     big = np.abs(So) >= 1e-3 * np.abs(So).max()
     assert np.all(np.abs(S - So)[big] <= 1e-12 * np.abs(So)[big])
     F.free(Y)
+    del omega, plan
+    gc.collect()

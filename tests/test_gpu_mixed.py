@@ -132,3 +132,6 @@ def test_mixed_workload_at_its_configured_size(backend):
     finally:
         for pm in (U, Y0, Y, Z):
             F.free(pm)
+        del src, plan
+        import gc
+        gc.collect()
