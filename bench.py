@@ -583,6 +583,8 @@ def kernels_only(rows, k, reps, vendor=False):
     timed("norm_fp64", lambda: backend.dot_accumulate(acc, A, None, False, work), 0, blk)
     timed("scale_fp64", lambda: backend.scale(B, 0.6 + 0.8j), 0, 2 * blk)
     timed("axpby_fp64", lambda: backend.axpby(B, A, 0.5, 1.0), 0, 3 * blk)
+    # The ghost fill of the block (written once, 16 B per entry).
+    timed("ghost_fp64", lambda: backend.ghost_fill(B, 0, 1, 7), 0, blk)
     # The stencils on one panel of their workloads (HBM-bound; ideal bytes:
     # the panel read once and the image written once): the 216^3 Helmholtz
     # operator on 32 columns and the 2^22-row tridiagonal on 32 columns.
