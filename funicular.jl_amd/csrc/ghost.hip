@@ -158,7 +158,10 @@ static void build_tables(ZigTables &z) {
 
 __constant__ ZigTables g_zig;
 
-__device__ inline double gh_normal(uint64_t key, uint64_t d) {
+// The draw-th normal of a column's stream, reading the ziggurat tables from
+// wt, f and kt (the constant-memory copy, or a workgroup's LDS copy of it).
+__device__ inline double gh_normal_from(const double *__restrict__ wt, const double *__restrict__ f,
+                                        const uint64_t *__restrict__ kt, uint64_t key, uint64_t d) {
   const double inv53 = 1.1102230246251565e-16;  // 2^-53
   double x = 0.0;
   for (uint64_t a = 0; a < 64; ++a) {
@@ -167,8 +170,8 @@ __device__ inline double gh_normal(uint64_t key, uint64_t d) {
     const int i = (int)(w0 & 0xFF);
     const bool neg = (w0 >> 8) & 1;
     const uint64_t u = w0 >> 11;
-    x = (double)u * g_zig.wt[i];
-    if (u < g_zig.kt[i]) return neg ? -x : x;
+    x = (double)u * wt[i];
+    if (u < kt[i]) return neg ? -x : x;
     if (i == 0) {
       const double u1 = (double)((gh_word(key, GH_LANE1, c) >> 11) + 1) * inv53;
       const double u2 = (double)((gh_word(key, GH_LANE2, c) >> 11) + 1) * inv53;
@@ -177,11 +180,29 @@ __device__ inline double gh_normal(uint64_t key, uint64_t d) {
       if (yt + yt > xt * xt) return neg ? -(GH_R + xt) : (GH_R + xt);
     } else {
       const double u1 = (double)(gh_word(key, GH_LANE1, c) >> 11) * inv53;
-      const double y = g_zig.f[i] + u1 * (g_zig.f[i + 1] - g_zig.f[i]);
+      const double y = f[i] + u1 * (f[i + 1] - f[i]);
       if (y < det_exp(-0.5 * (x * x))) return neg ? -x : x;
     }
   }
   return x;
+}
+
+__device__ inline double gh_normal(uint64_t key, uint64_t d) {
+  return gh_normal_from(g_zig.wt, g_zig.f, g_zig.kt, key, d);
+}
+
+// The first attempt's fast accept alone: true and the normal in v when the
+// draw's first word lands inside its layer's rectangle, as gh_normal_from's
+// first iteration decides; false otherwise, when the full routine is needed.
+__device__ inline bool gh_normal_fast(const double *__restrict__ wt, const uint64_t *__restrict__ kt,
+                                      uint64_t key, uint64_t d, double &v) {
+  const uint64_t w0 = gh_word(key, 0, d);
+  const int i = (int)(w0 & 0xFF);
+  const bool neg = (w0 >> 8) & 1;
+  const uint64_t u = w0 >> 11;
+  const double x = (double)u * wt[i];
+  v = neg ? -x : x;
+  return u < kt[i];
 }
 
 template <typename T, bool CPLX>
@@ -200,6 +221,74 @@ __global__ __launch_bounds__(256) void ghost_kernel(T *__restrict__ dst, int64_t
     } else {
       store_elem<T, false>(dst, i + c * ld, (T)gh_normal(key, (uint64_t)row), T(0));
     }
+  }
+}
+
+// The fill on a 1-D grid of (column, GH_ROWS-row chunk) workgroups, for fewer
+// than 2^31 rows and workgroups. Each thread owns GH_E entries, GH_ROWS / GH_E
+// rows apart. It takes every draw's first fast accept and stores the entries;
+// a draw that misses it (about 1 in 100: a wedge or tail test, or a retry)
+// sets a bit, and the thread then resolves its own missed draws with the full
+// routine, storing that part again. A wave so runs the slow path a couple of
+// times per GH_E * 128 draws instead of for almost every 128, since with one
+// entry per thread some lane of the wave nearly always needs it. The values
+// are gh_normal's: the fast accept is its first test, and a missed draw is
+// recomputed by it from the first attempt. The column key is computed once per
+// thread and the ziggurat tables are read from an LDS copy.
+constexpr int GH_E = 8;
+constexpr int GH_ROWS = 256 * GH_E;
+
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(256) void ghost_rows_kernel(T *__restrict__ dst, int64_t ld, int64_t row0,
+                                                         uint32_t nrows, DivU32 chunks, int64_t col0,
+                                                         uint64_t seed) {
+  __shared__ double s_wt[256];
+  __shared__ double s_f[257];
+  __shared__ uint64_t s_kt[256];
+  const int t = threadIdx.x;
+  s_wt[t] = g_zig.wt[t];
+  s_kt[t] = g_zig.kt[t];
+  s_f[t] = g_zig.f[t];
+  if (t == 0) s_f[256] = g_zig.f[256];
+  __syncthreads();
+  const uint32_t b = blockIdx.x;
+  const uint32_t c = divu32(b, chunks);
+  const uint32_t base = (b - c * chunks.d) * (uint32_t)GH_ROWS;
+  const uint64_t key = gh_colkey(seed, col0 + (int64_t)c);
+  constexpr int NP = CPLX ? 2 : 1;
+  T *out = dst + NP * (int64_t)c * ld;
+  const double scale = CPLX ? 0.70710678118654752 : 1.0;
+  uint32_t pending = 0;
+#pragma unroll
+  for (int j = 0; j < GH_E; ++j) {
+    const uint32_t i = base + (uint32_t)(j * 256 + t);
+    if (i < nrows) {
+      const uint64_t row = (uint64_t)(row0 + (int64_t)i);
+      if (CPLX) {
+        double re, im;
+        const bool okr = gh_normal_fast(s_wt, s_kt, key, 2 * row, re);
+        const bool oki = gh_normal_fast(s_wt, s_kt, key, 2 * row + 1, im);
+        pending |= (okr ? 0u : 1u) << (2 * j);
+        pending |= (oki ? 0u : 1u) << (2 * j + 1);
+        store_elem<T, true>(out, i, (T)(re * scale), (T)(im * scale));
+      } else {
+        double v;
+        const bool ok = gh_normal_fast(s_wt, s_kt, key, row, v);
+        pending |= (ok ? 0u : 1u) << j;
+        store_elem<T, false>(out, i, (T)v, T(0));
+      }
+    }
+  }
+  // At most NP * GH_E bits, one cleared per pass.
+  while (pending) {
+    const int bit = __builtin_ctz(pending);
+    pending &= pending - 1;
+    const int j = CPLX ? bit >> 1 : bit;
+    const int part = CPLX ? bit & 1 : 0;
+    const uint32_t i = base + (uint32_t)(j * 256 + t);
+    const uint64_t row = (uint64_t)(row0 + (int64_t)i);
+    const uint64_t d = CPLX ? 2 * row + (uint64_t)part : row;
+    out[NP * (int64_t)i + part] = (T)(gh_normal_from(s_wt, s_f, s_kt, key, d) * scale);
   }
 }
 
@@ -239,9 +328,33 @@ extern "C" int fhip_ghost_randn(void *dst, int type, int64_t ld, int64_t row0, i
   if (ld < nrows) return fail(FHIP_E_ARGUMENT, "fhip_ghost_randn: leading dimension smaller than the rows");
   int st = ensure_tables();
   if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t chunks = cdiv64(nrows, GH_ROWS);
+  if (nrows < (int64_t(1) << 31) && chunks * ncols < (int64_t(1) << 31)) {
+    const dim3 grid((unsigned)(chunks * ncols));
+    const DivU32 dc = make_divu32((uint32_t)chunks);
+    switch (type) {
+      case FHIP_C64:
+        hipLaunchKernelGGL((ghost_rows_kernel<double, true>), grid, dim3(256), 0, s, (double *)dst, ld, row0,
+                           (uint32_t)nrows, dc, col0, seed);
+        break;
+      case FHIP_C32:
+        hipLaunchKernelGGL((ghost_rows_kernel<float, true>), grid, dim3(256), 0, s, (float *)dst, ld, row0,
+                           (uint32_t)nrows, dc, col0, seed);
+        break;
+      case FHIP_F64:
+        hipLaunchKernelGGL((ghost_rows_kernel<double, false>), grid, dim3(256), 0, s, (double *)dst, ld, row0,
+                           (uint32_t)nrows, dc, col0, seed);
+        break;
+      case FHIP_F32:
+        hipLaunchKernelGGL((ghost_rows_kernel<float, false>), grid, dim3(256), 0, s, (float *)dst, ld, row0,
+                           (uint32_t)nrows, dc, col0, seed);
+        break;
+    }
+    return check_launch("ghost_rows_kernel");
+  }
   int64_t g = cdiv64(nrows * ncols, 256);
   if (g > 262144) g = 262144;
-  hipStream_t s = (hipStream_t)stream;
   switch (type) {
     case FHIP_C64:
       hipLaunchKernelGGL((ghost_kernel<double, true>), dim3((unsigned)g), dim3(256), 0, s, (double *)dst, ld, row0,
