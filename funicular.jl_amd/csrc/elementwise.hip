@@ -280,6 +280,28 @@ __global__ __launch_bounds__(256) void scale_kernel(T *__restrict__ Y, int64_t l
   }
 }
 
+// y <- a y + b entry by entry, the product formed as scale_kernel forms it
+// and the constant added after it, each part rounded once per operation.
+template <typename T, bool C>
+__global__ __launch_bounds__(256) void affine_kernel(T *__restrict__ Y, int64_t ldy, int64_t rows, T ar, T ai,
+                                                     int real_scalar, T br, T bi) {
+#pragma clang fp contract(off)
+  const int64_t j = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < rows; i += (int64_t)gridDim.x * 256) {
+    T yr, yi;
+    load_elem<T, T, C>(Y, i + j * ldy, true, yr, yi);
+    T pr, pi;
+    if (!C || real_scalar) {
+      pr = yr * ar;
+      pi = yi * ar;
+    } else {
+      pr = yr * ar - yi * ai;
+      pi = yr * ai + yi * ar;
+    }
+    store_elem<T, C>(Y, i + j * ldy, pr + br, pi + bi);
+  }
+}
+
 // Sum over a rows x cols block of conj(x) y (mode 0) or |x|^2 (mode 1), in
 // FP64. Workgroup (chunk, column) sums DOT_ROWS rows of one column, each
 // thread DOT_PER_THREAD of them at a stride of 256, then an LDS tree in a
@@ -582,6 +604,39 @@ extern "C" int fhip_scale(void *Y, int64_t ldy, int64_t rows, int64_t cols, doub
       break;
   }
   return check_launch("scale_kernel");
+}
+
+extern "C" int fhip_affine(void *Y, int64_t ldy, int64_t rows, int64_t cols, double a_re, double a_im,
+                           int real_scalar, double b_re, double b_im, int type, void *stream) {
+  if (!valid_type(type)) return fail(FHIP_E_UNSUPPORTED, "fhip_affine: unknown eltype");
+  if (rows < 0 || cols < 0) return fail(FHIP_E_ARGUMENT, "fhip_affine: negative dimension");
+  if (rows == 0 || cols == 0) return FHIP_OK;
+  if (!Y) return fail(FHIP_E_ARGUMENT, "fhip_affine: null pointer");
+  if (ldy < rows) return fail(FHIP_E_ARGUMENT, "fhip_affine: leading dimension smaller than the rows");
+  if (cols > 65535) return fail(FHIP_E_ARGUMENT, "fhip_affine: more than 65535 columns in one call");
+  int64_t gx = cdiv64(rows, 256);
+  if (gx > 4096) gx = 4096;
+  const dim3 grid((unsigned)gx, (unsigned)cols);
+  hipStream_t s = (hipStream_t)stream;
+  switch (type) {
+    case FHIP_F32:
+      hipLaunchKernelGGL((affine_kernel<float, false>), grid, dim3(256), 0, s, (float *)Y, ldy, rows, (float)a_re,
+                         0.0f, 1, (float)b_re, 0.0f);
+      break;
+    case FHIP_F64:
+      hipLaunchKernelGGL((affine_kernel<double, false>), grid, dim3(256), 0, s, (double *)Y, ldy, rows, a_re, 0.0,
+                         1, b_re, 0.0);
+      break;
+    case FHIP_C32:
+      hipLaunchKernelGGL((affine_kernel<float, true>), grid, dim3(256), 0, s, (float *)Y, ldy, rows, (float)a_re,
+                         (float)a_im, real_scalar, (float)b_re, (float)b_im);
+      break;
+    case FHIP_C64:
+      hipLaunchKernelGGL((affine_kernel<double, true>), grid, dim3(256), 0, s, (double *)Y, ldy, rows, a_re, a_im,
+                         real_scalar, b_re, b_im);
+      break;
+  }
+  return check_launch("affine_kernel");
 }
 
 extern "C" int fhip_dot_workspace_bytes(size_t *nbytes, int64_t rows, int64_t cols) {

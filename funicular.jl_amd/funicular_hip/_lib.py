@@ -88,6 +88,7 @@ _SIGNATURES = {
     "fhip_rowdot": [_vp, _vp, _i64, _vp, _i64, _i64, _i64, _i, _i, _vp],
     "fhip_axpby": [_vp, _i64, _vp, _i64, _i64, _i64, _d, _d, _d, _d, _i, _vp],
     "fhip_scale": [_vp, _i64, _i64, _i64, _d, _d, _i, _i, _vp],
+    "fhip_affine": [_vp, _i64, _i64, _i64, _d, _d, _i, _d, _d, _i, _vp],
     "fhip_trsm_apply": [_vp, _i64, _i64, _i64, _i, _i, _i, _vp, ctypes.c_size_t, _vp],
     "fhip_potrf_upper": [_vp, _i64, _i, _vp, _i64, _i, _i64, _vp, _vp],
     "fhip_gemm_nn_scaled": [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _d, _d, _d, _d, _i, _vp],

@@ -115,6 +115,37 @@ class DeviceArray:
     def size(self):
         return self.shape[0] * self.shape[1]
 
+    # In-place elementwise updates on the current stream, the broadcasts a
+    # foreachpanel closure applies to its buffer in Julia (panel .*= a,
+    # panel .+= X, panel .= a .* panel .+ b, panel .= v; src/executor.jl:241-267).
+    def __imul__(self, alpha):
+        _array_backend().scale(self, alpha)
+        return self
+
+    def __iadd__(self, other):
+        if isinstance(other, DeviceArray):
+            _array_backend().axpby(self, other, 1.0, 1.0)
+        else:
+            _array_backend().affine(self, 1.0, other)
+        return self
+
+    def __isub__(self, other):
+        if isinstance(other, DeviceArray):
+            _array_backend().axpby(self, other, -1.0, 1.0)
+        else:
+            _array_backend().affine(self, 1.0, -other)
+        return self
+
+    def affine_(self, alpha, beta):
+        """self <- alpha self + beta (Julia's self .= alpha .* self .+ beta)."""
+        _array_backend().affine(self, alpha, beta)
+        return self
+
+    def fill_(self, value):
+        """self <- value everywhere (Julia's self .= value)."""
+        _array_backend().fill(self, value)
+        return self
+
     @property
     def nbytes(self):
         return self.size * self.dtype.itemsize
@@ -601,6 +632,21 @@ class HIPBackend(DeviceBackend):
                  current_stream())
         return Y
 
+    def affine(self, Y, alpha, beta):
+        """Y <- alpha Y + beta entry by entry: the product as scale forms it,
+        then beta added, each part rounded once per operation (Julia's
+        Y .= alpha .* Y .+ beta)."""
+        real_scalar = not np.iscomplexobj(alpha)
+        a, b = complex(alpha), complex(beta)
+        if not np.issubdtype(Y.dtype, np.complexfloating) and (a.imag != 0 or b.imag != 0):
+            raise ArgumentError(f"cannot apply the complex coefficients {alpha}, {beta} to a real {Y.dtype} matrix")
+        code = eltype_code(Y.dtype)
+        for c0 in range(0, Y.cols, _MAX_GRID_COLS):
+            part = Y.view(None, (c0, min(Y.cols, c0 + _MAX_GRID_COLS)))
+            call("fhip_affine", part.ptr, part.ld, part.rows, part.cols, a.real, a.imag, 1 if real_scalar else 0,
+                 b.real, b.imag, code, current_stream())
+        return Y
+
     def dot_workspace_bytes(self, rows, cols):
         n = ctypes.c_size_t()
         call("fhip_dot_workspace_bytes", ctypes.byref(n), int(rows), int(min(cols, _MAX_GRID_COLS)))
@@ -676,6 +722,12 @@ def _workspace(backend, work, need):
 _pending_workspaces = []
 
 _backend = None
+
+
+def _array_backend():
+    """The backend a DeviceArray's own updates launch through: the one already
+    made, else the default device's."""
+    return _backend if _backend is not None else hip_backend()
 
 
 def hip_backend(device=0):

@@ -199,6 +199,26 @@ function LinearAlgebra.rmul!(Y::HIPMatrix{T}, α::Number) where {T}
     Y
 end
 
+# The broadcasts of a foreachpanel closure (`panel .*= a`, `panel .= a .* panel
+# .+ b`, `panel .= v`) have no device array package to compile them here; a
+# closure spells them as rmul!(panel, a), affine!(panel, a, b) and
+# fill!(panel, v), which give the same values as the broadcasts.
+function affine!(Y::HIPMatrix{T}, α::Number, β::Number) where {T}
+    a, b = ComplexF64(α), ComplexF64(β)
+    check(ccall((:fhip_affine, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Int64, Float64, Float64, Cint, Float64, Float64, Cint, Ptr{Cvoid}),
+        Y.ptr, Y.ld, size(Y, 1), size(Y, 2), real(a), imag(a), Cint(α isa Real), real(b), imag(b), eltcode(T),
+        current_stream()))
+    Y
+end
+
+function Base.fill!(Y::HIPMatrix{T}, v) where {T}
+    x = ComplexF64(v)
+    check(ccall((:fhip_fill, LIB[]), Cint, (Ptr{Cvoid}, Cint, Int64, Int64, Int64, Float64, Float64, Ptr{Cvoid}),
+        Y.ptr, eltcode(T), Y.ld, size(Y, 1), size(Y, 2), real(x), imag(x), current_stream()))
+    Y
+end
+
 function LinearAlgebra.axpy!(α::Number, X::HIPMatrix{T}, Y::HIPMatrix{T}) where {T}
     size(X) == size(Y) || throw(DimensionMismatch("axpy! between a $(size(X)) and a $(size(Y)) matrix"))
     a = ComplexF64(α)

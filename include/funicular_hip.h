@@ -244,6 +244,15 @@ int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int64_t rows, i
 int fhip_scale(void *Y, int64_t ldy, int64_t rows, int64_t cols, double a_re, double a_im,
                int real_scalar, int type, void *stream);
 
+/* Y <- alpha Y + beta for a rows x cols matrix, entry by entry: the product
+ * as fhip_scale forms it (real_scalar as there), then beta added, each part
+ * rounded once per operation (no fused multiply-add). At most 65535 columns
+ * per call. The broadcasts a foreachpanel closure applies to its device
+ * buffer (src/executor.jl:241-267): panel .= a .* panel .+ b
+ * (test/test_executor.jl:30-31); panel .= b is fhip_fill. */
+int fhip_affine(void *Y, int64_t ldy, int64_t rows, int64_t cols, double a_re, double a_im,
+                int real_scalar, double b_re, double b_im, int type, void *stream);
+
 /* Bytes of workspace fhip_dot needs for a rows x cols block. */
 int fhip_dot_workspace_bytes(size_t *nbytes, int64_t rows, int64_t cols);
 

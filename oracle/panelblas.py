@@ -177,6 +177,22 @@ def scale(Y, alpha):
     return out
 
 
+def affine(Y, alpha, beta):
+    """Julia's Y .= alpha .* Y .+ beta inside a foreachpanel closure
+    (src/executor.jl:241-267, test/test_executor.jl:30-31): the product as
+    scale forms it, then beta added to each part, one rounding per
+    operation."""
+    P = scale(Y, alpha)
+    if not np.iscomplexobj(P):
+        return P + P.dtype.type(np.real(beta))
+    R = P.real.dtype.type
+    b = complex(beta)
+    out = np.empty_like(P)
+    with np.errstate(invalid="ignore", over="ignore"):
+        out.real, out.imag = P.real + R(b.real), P.imag + R(b.imag)
+    return out
+
+
 def axpy(alpha, X, Y):
     """src/panelblas.jl:323-331: Y + alpha X, panel by panel (the panels are
     independent, so the order does not matter)."""

@@ -451,3 +451,57 @@ def test_project_by_runs_over_a_disk_backed_basis(backend, tmp_path):
         assert OB.isapprox(S, ref, OB.blastol(C16, N, k))
     assert F.disk_reads(Q) > before
     assert np.array_equal(F.to_matrix(Q), A)
+
+
+@pytest.mark.parametrize("w", (1, 4, 7, 23))
+@pytest.mark.parametrize("nb", (1, 2))
+def test_foreachpanel_closures_that_broadcast(backend, w, nb):
+    """The closures of test/test_executor.jl:25-37 and test/gpu_smoke.jl:79-84
+    on device buffers: panel .= 2 .* panel .+ T(j) (affine_), panel .*= 2,
+    panel .= T(j) (fill_), and panel .+= / .-= another buffer, each exactly
+    the oracle's, over a ragged last panel."""
+    rng = np.random.default_rng(980 + w)
+    N, k = 9, 23
+    A, B = randmatrix(rng, N, k), randmatrix(rng, N, k)
+    plan = make_plan(backend)
+    pm = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    F.foreachpanel(lambda j, panel: panel.affine_(2, j + 1), pm, nbuffers=nb)
+    index = np.array([[c // w + 1 for c in range(k)]] * N, dtype=C16)
+    expect = OB.affine(A, 2, 0) + index
+    assert np.array_equal(F.to_matrix(pm), expect)
+
+    def double(j, panel):
+        panel *= 2
+    F.foreachpanel(double, pm, nbuffers=nb)
+    assert np.array_equal(F.to_matrix(pm), OB.scale(expect, 2.0))
+    F.foreachpanel(lambda j, panel: panel.fill_(j + 1), pm, nbuffers=nb)
+    assert np.array_equal(F.to_matrix(pm), index)
+    X = F.PanelMatrix.from_array(B, plan=plan, w=w)
+
+    def add(j, xb, yb):
+        yb += xb
+        yb -= 0.5
+    sweep(add, readpanels(X), updatepanels(pm), nbuffers=nb)
+    assert np.array_equal(F.to_matrix(pm), OB.affine(index + B, 1.0, -0.5))
+    # A complex scalar: the schoolbook product, then the constant.
+    a, b = 0.3 - 1.7j, 1.25 + 0.5j
+    F.foreachpanel(lambda j, panel: panel.affine_(a, b), X, nbuffers=nb)
+    assert np.array_equal(F.to_matrix(X), OB.affine(B, a, b))
+
+
+def test_broadcast_closure_over_narrowed_storage(backend):
+    # test/gpu_smoke.jl:115-122: the closure sees ComplexF64 buffers; the
+    # results are narrowed to the ComplexF32 host tier on the way back.
+    rng = np.random.default_rng(990)
+    N, k, w = 300, 24, 4
+    A = randmatrix(rng, N, k)
+    plan = make_plan(backend, host_eltype=np.complex64)
+    Z = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    A32 = A.astype(np.complex64).astype(C16)
+    assert np.array_equal(F.to_matrix(Z), A32)
+
+    def double(j, panel):
+        assert panel.dtype == np.dtype(C16)
+        panel *= 2
+    F.foreachpanel(double, Z)
+    assert np.array_equal(F.to_matrix(Z), 2 * A32)

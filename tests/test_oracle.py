@@ -144,6 +144,22 @@ def test_panelwise_cholqr_against_the_row_sweeps():
     assert np.linalg.norm(Q @ Rq - B) <= 1e-10 * np.linalg.norm(B)
 
 
+def test_affine_known_answers():
+    # (1 + 2i)(3 - i) + (0.5 - 5i) = 5.5 exactly; 2 y + 3 on reals.
+    assert OB.affine(np.array([1 + 2j]), 3 - 1j, 0.5 - 5j)[0] == 5.5 + 0j
+    assert OB.affine(np.array([1.5, -2.0]), 2.0, 3.0).tolist() == [6.0, -1.0]
+    # The product is rounded before the constant is added: (1 + 2^-27)^2 - 1
+    # is 2^-26 that way, and 2^-26 + 2^-54 had the two been fused.
+    t = 1.0 + 2.0 ** -27
+    assert OB.affine(np.array([t]), t, -1.0)[0] == 2.0 ** -26
+    assert OB.affine(np.array([t + 0j]), t, -1.0)[0] == complex(2.0 ** -26, 0.0)
+    # alpha = 1, beta = 0 leaves the values; beta = 0 is scale.
+    rng = np.random.default_rng(12)
+    Y = rng.standard_normal(32) + 1j * rng.standard_normal(32)
+    assert np.array_equal(OB.affine(Y, 1.0, 0.0), Y)
+    assert np.array_equal(OB.affine(Y, 0.3 - 1.7j, 0.0), OB.scale(Y, 0.3 - 1.7j))
+
+
 def test_scale_known_answers():
     # (1 + 2i)(3 - i) = 5 + 5i exactly; real scalars scale each part.
     assert OB.scale(np.array([1 + 2j]), 3 - 1j)[0] == 5 + 5j
