@@ -43,6 +43,17 @@ def test_argument_errors_need_no_device():
     with pytest.raises(_lib.LibraryError) as e:
         _lib.call("fhip_convert_copy", None, 0, 1, None, 3, 1, 1, 1, None)
     assert "null pointer" in str(e.value)
+    # The elementwise updates: empty blocks are no-ops, bad shapes are errors.
+    assert lib.fhip_affine(None, 1, 0, 5, 2.0, 0.0, 1, 1.0, 0.0, 3, None) == 0
+    assert lib.fhip_affine(None, 4, 4, 1, 2.0, 0.0, 1, 1.0, 0.0, 3, None) == _lib.FHIP_E_ARGUMENT
+    assert lib.fhip_affine(None, 4, -1, 1, 2.0, 0.0, 1, 1.0, 0.0, 3, None) == _lib.FHIP_E_ARGUMENT
+    assert lib.fhip_affine(None, 4, 4, 1, 2.0, 0.0, 1, 1.0, 0.0, 9, None) == _lib.FHIP_E_UNSUPPORTED
+    # A leading dimension short of the rows is refused before any device work
+    # (the address is never used).
+    assert lib.fhip_scale(ctypes.c_void_p(256), 2, 4, 1, 2.0, 0.0, 1, 3, None) == _lib.FHIP_E_ARGUMENT
+    assert b"leading dimension" in lib.fhip_last_error()
+    assert lib.fhip_affine(ctypes.c_void_p(256), 2, 4, 1, 2.0, 0.0, 1, 1.0, 0.0, 3, None) == _lib.FHIP_E_ARGUMENT
+    assert b"leading dimension" in lib.fhip_last_error()
 
 
 def test_workspace_depends_on_shape_only():
