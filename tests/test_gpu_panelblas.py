@@ -505,3 +505,39 @@ def test_broadcast_closure_over_narrowed_storage(backend):
         panel *= 2
     F.foreachpanel(double, Z)
     assert np.array_equal(F.to_matrix(Z), 2 * A32)
+
+
+@pytest.mark.parametrize("resident", (False, True))
+def test_no_device_allocation_after_warmup(backend, resident):
+    """test/test_cuda.jl:100-111 on the library's own counter: after one run to
+    fill the pools, a data sweep, a copying sweep, panelmul, gram, cholqr2 and
+    project allocate no device memory at all (every workspace and staging
+    buffer comes from the plan's pool, and no kernel stages scalars)."""
+    rng = np.random.default_rng(1000)
+    N, k, w = 4000, 48, 8
+    A = randmatrix(rng, N, k)
+    plan = make_plan(backend, device_budget=256 << 20, device_resident=resident)
+    G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, backend=backend)
+    X = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    Z = F.PanelMatrix.from_array(A, plan=plan, w=w)
+
+    def double(j, panel):
+        panel *= 2
+
+    def copying(j, src, dst):
+        backend.convert_copy(dst, src)
+    steps = [lambda: sweep(double, updatepanels(X)),
+             lambda: sweep(copying, readpanels(X), writepanels(Y)),
+             lambda: F.panelmul(Y, G, X),
+             lambda: F.gram(X),
+             lambda: F.cholqr2(Z),
+             lambda: F.project(Z, G)]
+    for fn in steps:
+        fn()
+    backend.sync_all()
+    for fn in steps:
+        before = F._lib.counters()["device_allocs"]
+        fn()
+        backend.sync_all()
+        assert F._lib.counters()["device_allocs"] == before
