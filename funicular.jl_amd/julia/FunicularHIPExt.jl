@@ -167,10 +167,28 @@ end
 function rdiv_upper!(::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}) where {T}
     n = Ref{Csize_t}(0)
     check(ccall((:fhip_trsm_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Cint), n, size(R, 1), eltcode(T)))
-    work = workspace(n[])
+    work = workspace(n[]; which=TRSM_WORK)
     check(ccall((:fhip_trsm_runn, LIB[]), Cint,
         (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
         Y.ptr, Y.ld, size(Y, 1), size(Y, 2), R.ptr, R.ld, eltcode(T), eltcode(T), work.ptr, n[], current_stream()))
+    Y
+end
+
+# With the block index passed to rdiv_upper! by rdiv_rows! (src/panelblas.jl:157-169;
+# the default method for other backends drops it), R is folded into the solve's
+# workspace once, on the sweep's first block, and the later blocks reuse it: the
+# solve's workspace is its own, so a Gram between the blocks (the fused next
+# pass) does not overwrite the folded factor. Every block of one rdiv_rows!
+# runs on the one compute stream, so the fold is complete before a later block
+# reads it.
+function rdiv_upper!(b::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}, block::Integer) where {T}
+    block == 1 && return rdiv_upper!(b, Y, R)
+    n = Ref{Csize_t}(0)
+    check(ccall((:fhip_trsm_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Cint), n, size(R, 1), eltcode(T)))
+    work = workspace(n[]; which=TRSM_WORK)
+    check(ccall((:fhip_trsm_apply, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+        Y.ptr, Y.ld, size(Y, 1), size(Y, 2), eltcode(T), eltcode(T), Cint(0), work.ptr, n[], current_stream()))
     Y
 end
 
@@ -252,8 +270,11 @@ end
 # cholesky_upper), the plain factorization runs on the device; a bad pivot
 # hands G to cholesky_upper on the host, which retries and shifts as the
 # reference does.
+const INFO = Ref{Any}(nothing)
+
 function factor_gram!(b::HIPBackend, R::HIPMatrix{T}, G::HIPMatrix{T}, N::Integer) where {T}
-    info = alloc_device(b, Cint, (1, 1))
+    INFO[] === nothing && (INFO[] = alloc_device(b, Cint, (1, 1)))  # one status word, kept for every pass
+    info = INFO[]
     check(ccall((:fhip_potrf_upper, LIB[]), Cint,
         (Ptr{Cvoid}, Int64, Cint, Ptr{Cvoid}, Int64, Cint, Int64, Ptr{Cvoid}, Ptr{Cvoid}),
         R.ptr, R.ld, eltcode(T), G.ptr, G.ld, eltcode(T), size(G, 1), info.ptr, current_stream()))
@@ -270,17 +291,90 @@ function factor_gram!(b::HIPBackend, R::HIPMatrix{T}, G::HIPMatrix{T}, N::Intege
     shifted
 end
 
-# A per-backend workspace that grows to the largest request and is reused;
-# the first sweep of a shape allocates it, later ones do not.
+# Workspaces that grow to the largest request and are reused: the first
+# sweep of a shape allocates, later ones do not. The solve keeps its own (it
+# holds the folded factor between row blocks). Before a smaller buffer is
+# dropped the device is drained, since a kernel already queued may still be
+# using it and dropping it lets the finalizer free it.
 const WORK = Ref{Any}(nothing)
-function workspace(n)
-    w = WORK[]
+const TRSM_WORK = Ref{Any}(nothing)
+function workspace(n; which=WORK)
+    w = which[]
     if w === nothing || w.nbytes < n
-        w = alloc_device(HIPBackend(), UInt8, (max(Int(n), 1), 1))
-        WORK[] = (ptr=w.ptr, nbytes=Int(n), owner=w)
-        w = WORK[]
+        w === nothing || check(ccall((:fhip_device_sync, LIB[]), Cint, ()))
+        buf = alloc_device(HIPBackend(), UInt8, (max(Int(n), 1), 1))
+        which[] = (ptr=buf.ptr, nbytes=Int(n), owner=buf)
+        w = which[]
     end
     w
+end
+
+# The two stencils of the benchmark configurations as Funicular operators
+# (src/operator.jl:1-25: size, eltype, adjoint, mul!, and the panel_capable and
+# ishermitian_op traits), applied to whole device panels by the library:
+# the 3-D 7-point periodic Helmholtz stencil on n1 x n2 x n3 (linear index
+# i + n1 (j + n2 l)), and the periodic tridiagonal with an optional real
+# diagonal kept on the device.
+struct HelmholtzStencil{T}
+    n::NTuple{3,Int}
+    d::ComplexF64
+    o::ComplexF64
+    adjoint::Bool
+end
+HelmholtzStencil{T}(n1, n2, n3, d, o=-1.0) where {T} = HelmholtzStencil{T}((Int(n1), Int(n2), Int(n3)), d, o, false)
+Base.size(G::HelmholtzStencil) = (prod(G.n), prod(G.n))
+Base.eltype(::HelmholtzStencil{T}) where {T} = T
+LinearAlgebra.adjoint(G::HelmholtzStencil{T}) where {T} = HelmholtzStencil{T}(G.n, G.d, G.o, !G.adjoint)
+Funicular.panel_capable(::HelmholtzStencil) = true
+Funicular.ishermitian_op(G::HelmholtzStencil) = imag(G.d) == 0 && imag(G.o) == 0
+function LinearAlgebra.mul!(Y::HIPMatrix{T}, G::HelmholtzStencil{T}, X::HIPMatrix{T}) where {T}
+    size(X) == size(Y) && size(X, 1) == size(G, 1) ||
+        throw(DimensionMismatch("a $(size(G)) stencil cannot map $(size(X)) onto $(size(Y))"))
+    check(ccall((:fhip_stencil_helmholtz3d, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Int64, Int64, Float64, Float64, Float64, Float64, Cint,
+         Cint, Ptr{Cvoid}),
+        Y.ptr, Y.ld, X.ptr, X.ld, G.n[1], G.n[2], G.n[3], size(X, 2), real(G.d), imag(G.d), real(G.o), imag(G.o),
+        Cint(G.adjoint), eltcode(T), current_stream()))
+    Y
+end
+
+struct TridiagonalStencil{T}
+    N::Int
+    d::ComplexF64
+    o::ComplexF64
+    diag::Union{Nothing,HIPMatrix{Float64}}
+    adjoint::Bool
+end
+TridiagonalStencil{T}(N, d, o, diag=nothing) where {T} = TridiagonalStencil{T}(Int(N), d, o, diag, false)
+Base.size(G::TridiagonalStencil) = (G.N, G.N)
+Base.eltype(::TridiagonalStencil{T}) where {T} = T
+LinearAlgebra.adjoint(G::TridiagonalStencil{T}) where {T} = TridiagonalStencil{T}(G.N, G.d, G.o, G.diag, !G.adjoint)
+Funicular.panel_capable(::TridiagonalStencil) = true
+Funicular.ishermitian_op(G::TridiagonalStencil) = imag(G.d) == 0 && imag(G.o) == 0
+function LinearAlgebra.mul!(Y::HIPMatrix{T}, G::TridiagonalStencil{T}, X::HIPMatrix{T}) where {T}
+    size(X) == size(Y) && size(X, 1) == G.N ||
+        throw(DimensionMismatch("a $(size(G)) stencil cannot map $(size(X)) onto $(size(Y))"))
+    diag = G.diag === nothing ? C_NULL : G.diag.ptr
+    check(ccall((:fhip_stencil_tridiag, LIB[]), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Float64, Float64, Float64, Float64, Ptr{Cvoid}, Cint,
+         Cint, Ptr{Cvoid}),
+        Y.ptr, Y.ld, X.ptr, X.ld, G.N, size(X, 2), real(G.d), imag(G.d), real(G.o), imag(G.o), diag,
+        Cint(G.adjoint), eltcode(T), current_stream()))
+    Y
+end
+
+# Ghost panels generated straight into a device buffer (rows row0+1 ..
+# row0+size(dst, 1) of columns col0 .. col0+size(dst, 2)-1), for a hook in the
+# ghost tier's materialize! whose default is the reference's host generation
+# (src/ghosts.jl:97-122) and an upload. The library's stream is its own
+# counter-based one, not Julia's Xoshiro stream, so a GhostPanels matrix has to
+# opt in to it: the values are standard complex normals with the same
+# contract (they depend on (seed, col) alone), but not the same numbers.
+function fill_ghost!(::HIPBackend, dst::HIPMatrix{T}, row0::Integer, col0::Integer, seed::Integer) where {T}
+    check(ccall((:fhip_ghost_randn, LIB[]), Cint,
+        (Ptr{Cvoid}, Cint, Int64, Int64, Int64, Int64, Int64, UInt64, Ptr{Cvoid}),
+        dst.ptr, eltcode(T), dst.ld, row0, size(dst, 1), col0, size(dst, 2), UInt64(seed), current_stream()))
+    dst
 end
 
 end # module FunicularHIPExt

@@ -73,3 +73,58 @@ def test_ghost_tables_match_the_oracle():
     out = ctypes.c_uint64()
     _lib.call("fhip_ghost_tables_checksum", ctypes.byref(out))
     assert out.value == OG.tables_checksum()
+
+
+def _split_top(text):
+    parts, depth, cur = [], 0, ""
+    for ch in text:
+        if ch in "([{":
+            depth += 1
+        elif ch in ")]}":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    parts.append(cur)
+    return [p for p in (q.strip() for q in parts) if p]
+
+
+def _closing(text, i):
+    """Index of the parenthesis that closes the one at text[i]."""
+    depth = 0
+    for j in range(i, len(text)):
+        if text[j] == "(":
+            depth += 1
+        elif text[j] == ")":
+            depth -= 1
+            if depth == 0:
+                return j
+    raise AssertionError("unbalanced parentheses")
+
+
+def test_julia_binding_matches_the_header():
+    # The Julia extension cannot run here (no Julia), so its ccalls are held to
+    # the C prototypes: each names a declared entry point, with as many argument
+    # types as the prototype has parameters, and as many arguments as types.
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    header = open(os.path.join(root, "include", "funicular_hip.h")).read()
+    protos = {}
+    for m in re.finditer(r"\b(?:int|const char \*)\s*(fhip_\w+)\s*\(([^;]*?)\)\s*;", header, re.S):
+        params = [p for p in _split_top(m.group(2).replace("\n", " ")) if p != "void"]
+        protos[m.group(1)] = len(params)
+    jl = open(os.path.join(root, "funicular.jl_amd", "julia", "FunicularHIPExt.jl")).read()
+    calls = list(re.finditer(r"ccall\(\(:(fhip_\w+),\s*LIB\[\]\),\s*\w+,\s*\(", jl))
+    assert len(calls) > 20
+    for m in calls:
+        name = m.group(1)
+        t0 = m.end() - 1
+        t1 = _closing(jl, t0)
+        types = _split_top(jl[t0 + 1:t1])
+        a1 = _closing(jl, m.start() + len("ccall"))
+        args = _split_top(jl[t1 + 1:a1])
+        assert name in protos, name
+        assert len(types) == protos[name], (name, len(types), protos[name])
+        assert len(args) == len(types), (name, len(args), len(types))
