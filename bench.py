@@ -371,7 +371,7 @@ def disk_bench(backend, scratch):
     cholqr2_panelwise orthonormalizes it panel pair by panel pair (the row
     sweeps of cholqr2 would need all eight panels host resident); the same
     steps with the whole basis in host memory give the cost of the tier.
-    The scratch file sits on the box's local disk, behind the page cache."""
+    The scratch file sits on the machine's local disk, behind the page cache."""
     import funicular_hip as F
     from funicular_hip.executor import readpanels, sweep
     N, k, w = 1 << 22, 256, 32
@@ -618,7 +618,7 @@ def streamed(args, phase_gap=0.0):
     import funicular_hip as F
     from funicular_hip.executor import readpanels, rowtraversal, sweep
     backend = F.hip_backend()
-    # The GPU boxes this was measured on cap page-locked host memory at 64 GiB,
+    # The machines this was measured on cap page-locked host memory at 64 GiB,
     # below the 216^3 x 512 basis (82.6 GB); the streamed run takes the largest
     # grid whose basis fits (180^3 rows: 47.8 GB) under a 60 GiB host budget.
     n = args.stream_grid
