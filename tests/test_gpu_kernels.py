@@ -401,3 +401,23 @@ def test_fp32_products_over_tile_widths(backend, m, k, stored):
     Xr = OB.rdiv_upper(A64, R)
     assert np.linalg.norm(X - Xr) <= 1e-4 * np.linalg.norm(Xr)
     assert np.linalg.norm(X.conj().T @ X - np.eye(k)) <= 1e-3
+
+
+def test_ghost_fill_matches_the_fixture(backend):
+    """The device fill against tests/golden/ghost_stream.json (the oracle's
+    stream, committed): rows from 0 and from 1,000,000 of columns 1, 2 and 300
+    under two seeds, ComplexF64 and Float64, bit for bit."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ghost_stream.json")
+    fx = json.load(open(path))
+    for case in fx["cases"]:
+        n = case["rows"]
+        for col in fx["cols"]:
+            d = backend.alloc_device(np.complex128, (n, 1))
+            backend.ghost_fill(d, case["row0"], col, case["seed"])
+            want = np.array([complex(a, b) for a, b in case["complex128"][str(col)]])
+            assert np.array_equal(backend.to_host(d)[:, 0], want)
+            r = backend.alloc_device(np.float64, (n, 1))
+            backend.ghost_fill(r, case["row0"], col, case["seed"])
+            assert np.array_equal(backend.to_host(r)[:, 0], np.array(case["float64"][str(col)]))

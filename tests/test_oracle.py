@@ -278,3 +278,23 @@ def test_ghost_stream_is_standard_normal_and_keyed_by_column():
     assert not np.array_equal(OG.matrix(np.complex128, 100, 4, 0xDEADBEEF + 1), A[:100])
     R = OG.matrix(np.float64, 200000, 1, 7)
     assert abs(R.var() - 1) < 0.01
+
+
+def _ghost_fixture():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ghost_stream.json")
+    return json.load(open(path))
+
+
+def test_ghost_stream_matches_its_fixture():
+    # tests/golden/ghost_stream.json, written by tests/golden/make_ghost_fixture.py,
+    # holds the stream's values; a change to any bit of it shows up here.
+    fx = _ghost_fixture()
+    for case in fx["cases"]:
+        for col in fx["cols"]:
+            z = OG.ghost(np.complex128, case["row0"], case["rows"], col, 1, case["seed"])[:, 0]
+            want = np.array([complex(a, b) for a, b in case["complex128"][str(col)]])
+            assert np.array_equal(z, want)
+            x = OG.ghost(np.float64, case["row0"], case["rows"], col, 1, case["seed"])[:, 0]
+            assert np.array_equal(x, np.array(case["float64"][str(col)]))
