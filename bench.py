@@ -841,10 +841,11 @@ def main():
     record = {
         "metric": METRIC, "value": value, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64 (complex)", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"cholqr2 of a device-resident {N}x{k} ComplexF64 basis: {n}^3 periodic "
                                f"Helmholtz stencil applied to GhostPanels(seed 0xdeadbeef), panels of {w}",
-                   "N": N, "k": k, "panel_width": w, "row_block": F.row_block_height(N, k, w)},
+                   "N": N, "k": k, "panel_width": w, "row_block": F.row_block_height(N, k, w),
+                   "element": "ComplexF64 (FP64 arithmetic on real and imaginary parts)"},
         "roofline": {"bound": "mfma", "kernel": "gram_partial_kernel + gram_reduce_kernel (FP64)",
                      "achieved": gram64["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": gram64["tflops"] / FP64_PEAK_TFLOPS,
