@@ -243,13 +243,13 @@ def adjoint_bench(backend):
         sweep(lambda _, a, b: backend.axpby(b, a, 1.0, 1.0), readpanels(dQ), updatepanels(Qbar))
         return pb_qr(Qbar, None), dbar, Qbar
 
-    # One untimed round warms the pools and the allocator, then the best of
-    # three timed rounds of each direction.
+    # One untimed repetition warms the pools and the allocator, then the best
+    # of three timed repetitions of each direction.
     import gc
     fwd, bwd = [], []
     R = S = pb_qr = pb_pr = Ybar = Qbar = None
     for rep in range(4):
-        # The previous round's matrices are released before the clock starts.
+        # The previous repetition's matrices are released before the clock starts.
         del R, S, pb_qr, pb_pr
         for pm in (Ybar, Qbar):
             if pm is not None:
@@ -432,7 +432,7 @@ def disk_bench(backend, scratch):
 def range_finder_bench(backend, scratch, n=160, k=256, w=32, iterations=2):
     """The disk-tier randomized range finder at a reduced size: G = the 3-D
     7-point Helmholtz stencil on an n^3 grid, Omega = GhostPanels, Y = G Omega,
-    then `iterations` rounds of (Z = G^H Y, orthonormalize Z, Y = G Z,
+    then `iterations` power steps of (Z = G^H Y, orthonormalize Z, Y = G Z,
     orthonormalize Y), every orthonormalization cholqr2_panelwise with the
     mixed first pass. Y and Z live on the scratch disk behind a host budget of
     4 of their 8 panels, so the row sweeps of cholqr2 could not run at all.
