@@ -83,14 +83,19 @@ function alloc_host_slab(::HIPBackend, nbytes::Integer)
     aligned_slab(owner, Ptr{UInt8}(p[]), nbytes)
 end
 
-struct HIPQueue
+# make_sweep_queues runs once per sweep (src/executor.jl:193), so each queue
+# owns its stream and releases it when collected, as CUDA.jl's streams do.
+# Work still queued on the stream completes before its resources go.
+mutable struct HIPQueue
     stream::Ptr{Cvoid}
 end
 
 function make_queue(::HIPBackend)
     s = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:fhip_stream_create, LIB[]), Cint, (Ptr{Ptr{Cvoid}},), s))
-    HIPQueue(s[])
+    q = HIPQueue(s[])
+    finalizer(x -> ccall((:fhip_stream_destroy, LIB[]), Cint, (Ptr{Cvoid},), x.stream), q)
+    q
 end
 
 mutable struct HIPEvent
