@@ -139,10 +139,14 @@ def summarize(samples, flops_fn):
             "flops_per_launch": float(flops.mean()), "ms_per_row": float(ms.sum()) / rows}
 
 
-def cpu_baseline(k, w, rows, repeats=2):
+def cpu_baseline(k, w, rows, repeats=3):
     """The oracle's cholqr2 (the reference CPU backend's algorithm: row-block
     Grams, host Cholesky, row-block triangular solves, FP64 LAPACK/BLAS) timed
-    on this host's cores over a bounded sample of rows of the same workload."""
+    on this host's cores over a bounded sample of rows of the same workload:
+    one untimed call, then the best of `repeats`, end to end and per operation
+    (gram, the Cholesky and rdiv_rows, as the reference's benchmarks report
+    them, benchmark/common.jl:12-15). project is timed once: its p (p + 1)
+    steps take several times a whole cholqr2 on the CPU."""
     from oracle import ghost as OG
     from oracle import operators as OO
     from oracle import panelblas as OB
@@ -156,17 +160,33 @@ def cpu_baseline(k, w, rows, repeats=2):
     n2 = 128
     n3 = rows // (n1 * n2)
     N = n1 * n2 * n3
-    Y0 = OB.panelmul(lambda X: OO.helmholtz_apply(X, (n1, n2, n3), OO.helmholtz_diagonal()),
-                     OG.matrix(np.complex128, N, k, 0xDEADBEEF), w)
-    best = None
-    for _ in range(repeats):
-        t = time.perf_counter()
-        OB.cholqr2(Y0, w)
-        dt = time.perf_counter() - t
-        best = dt if best is None else min(best, dt)
+    apply = lambda X: OO.helmholtz_apply(X, (n1, n2, n3), OO.helmholtz_diagonal())
+    Y0 = OB.panelmul(apply, OG.matrix(np.complex128, N, k, 0xDEADBEEF), w)
+
+    def best_of(fn, n=repeats, warm=True):
+        if warm:
+            fn()
+        best = None
+        for _ in range(n):
+            t = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return best
+
+    best = best_of(lambda: OB.cholqr2(Y0, w))
+    G = OB.gram(Y0, w=w)
+    R, _ = OB.cholesky_upper(G, N, k)
+    Q, _ = OB.cholqr2(Y0, w)
+    per_op = {"gram_s": best_of(lambda: OB.gram(Y0, w=w)),
+              "cholesky_s": best_of(lambda: OB.cholesky_upper(G, N, k)),
+              "rdiv_rows_s": best_of(lambda: OB.rdiv_rows(Y0, R, w)),
+              "project_s": best_of(lambda: OB.project(Q, apply, w), n=1, warm=False)}
     return {"value": N / best, "unit": "rows/s", "cores": int(threads), "kind": "port",
             "sample": f"oracle cholqr2 (numpy/LAPACK FP64, reference step order) of a {N}x{k} ComplexF64 "
-                      f"Helmholtz-applied ghost basis, panels of {w}, best of {repeats}: {best:.2f} s"}
+                      f"Helmholtz-applied ghost basis, panels of {w}, one untimed call then best of "
+                      f"{repeats}: {best:.2f} s",
+            "per_operation": {key: round(v, 4) for key, v in per_op.items()}}
 
 
 def residual(backend, Y, R, G, omega):

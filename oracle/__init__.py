@@ -15,7 +15,8 @@ Contents, each function citing the reference file:line it follows:
   ghost.c       the ghost-panel stream (counter-based SplitMix64 + 256-layer
                 ziggurat with fixed-sequence exp/log), built by build.py
   ghost.py      ctypes wrapper of ghost.c
-  baseline.py   the timed CPU baseline (bench.py's cpu_baseline leg)
+  adjoint.py    the adjoints of cholqr2 and project (dense FP64)
+  build.py      compiles ghost.c into _build/
 
 Pinning (what makes each function trustworthy; see tests/test_oracle_*.py):
   plan.py       the reference's literal known answers (test/test_plan.jl:18-38,
