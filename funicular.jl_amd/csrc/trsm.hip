@@ -27,6 +27,7 @@
 // kappa(R).
 #include "fhip_internal.hpp"
 
+#include <mutex>
 #include <type_traits>
 
 namespace fhip {
@@ -440,6 +441,71 @@ static int launch_trsm(TS *Y, int64_t ldy, int64_t m, int k, const TS *R, int64_
   return launch_wide<TS, T, CPLX, CT>(Y, ldy, m, k, M, stream);
 }
 
+// Checks shared by fhip_trsm_runn and fhip_trsm_runn_noalloc, then the solve
+// with the folded factor in M (m_bytes long). `name` labels the messages.
+static int trsm_runn_checked(const char *name, void *Y, int64_t ldy, int64_t m, int64_t k, const void *R,
+                             int64_t ldr, int type, int compute, void *M, size_t m_bytes, hipStream_t s) {
+  if (!valid_type(type) || !valid_type(compute))
+    return fail(FHIP_E_UNSUPPORTED, "%s: unknown type %d / %d", name, type, compute);
+  if (!(compute == type || (type == FHIP_C64 && compute == FHIP_C32)))
+    return fail(FHIP_E_UNSUPPORTED, "%s: no kernel for storage type %d with compute type %d", name, type, compute);
+  if (m < 0 || k < 0) return fail(FHIP_E_ARGUMENT, "%s: negative dimension", name);
+  if (k == 0) return FHIP_OK;
+  if (k > (1 << 16)) return fail(FHIP_E_ARGUMENT, "%s: too many columns", name);
+  if (!Y || !R) return fail(FHIP_E_ARGUMENT, "%s: null matrix pointer", name);
+  if (ldr < k || (m > 0 && ldy < m))
+    return fail(FHIP_E_ARGUMENT, "%s: leading dimension smaller than the matrix", name);
+  size_t need = 0;
+  fhip_trsm_workspace_bytes(&need, k, compute);
+  if (!M || m_bytes < need)
+    return fail(FHIP_E_WORKSPACE, "%s: workspace of %zu bytes is smaller than the %zu needed", name, m_bytes, need);
+  const int ik = (int)k;
+  if (type == FHIP_C64 && compute == FHIP_C32)
+    return launch_trsm<double, float, true>((double *)Y, ldy, m, ik, (const double *)R, ldr, (float *)M, s);
+  switch (type) {
+    case FHIP_C64:
+      return launch_trsm<double, double, true>((double *)Y, ldy, m, ik, (const double *)R, ldr, (double *)M, s);
+    case FHIP_C32:
+      return launch_trsm<float, float, true>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)M, s);
+    case FHIP_F64:
+      return launch_trsm<double, double, false>((double *)Y, ldy, m, ik, (const double *)R, ldr, (double *)M, s);
+    case FHIP_F32:
+      return launch_trsm<float, float, false>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)M, s);
+  }
+  return fail(FHIP_E_UNSUPPORTED, "%s: unknown type %d", name, type);
+}
+
+// The folded factor of fhip_trsm_runn_noalloc lives here: a device array of
+// fixed size, one copy per GPU, held until the process ends. 16 MiB is the
+// factor of k = 1024 in FP64 complex (kp^2 x 16 bytes). Host code must never
+// pass the array's name to a kernel or a copy: it is a host-side stand-in,
+// not a device address. scratch_address() resolves the address on the
+// current device once and keeps it.
+constexpr size_t TRSM_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[TRSM_SCRATCH_BYTES];
+
+constexpr int TRSM_SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_lock;
+static void *g_scratch_ptr[TRSM_SCRATCH_DEVICES];
+
+static int scratch_address(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= TRSM_SCRATCH_DEVICES)
+    return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn_noalloc: device index %d out of range", dev);
+  std::lock_guard<std::mutex> guard(g_scratch_lock);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(trsm scratch)");
+    if (!p) return fail(FHIP_E_HIP, "hipGetSymbolAddress(trsm scratch) returned a null address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FHIP_OK;
+}
+
 }  // namespace fhip
 
 using namespace fhip;
@@ -456,37 +522,32 @@ extern "C" int fhip_trsm_workspace_bytes(size_t *nbytes, int64_t k, int compute)
 
 extern "C" int fhip_trsm_runn(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr,
                               int type, int compute, void *work, size_t work_bytes, void *stream) {
-  if (!valid_type(type) || !valid_type(compute))
-    return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runn: unknown type %d / %d", type, compute);
-  if (!(compute == type || (type == FHIP_C64 && compute == FHIP_C32)))
-    return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runn: no kernel for storage type %d with compute type %d", type,
-                compute);
-  if (m < 0 || k < 0) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: negative dimension");
-  if (k == 0) return FHIP_OK;
-  if (k > (1 << 16)) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: too many columns");
-  if (!Y || !R) return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: null matrix pointer");
-  if (ldr < k || (m > 0 && ldy < m))
-    return fail(FHIP_E_ARGUMENT, "fhip_trsm_runn: leading dimension smaller than the matrix");
-  size_t need = 0;
-  fhip_trsm_workspace_bytes(&need, k, compute);
-  if (!work || work_bytes < need)
-    return fail(FHIP_E_WORKSPACE, "fhip_trsm_runn: workspace of %zu bytes is smaller than the %zu needed",
-                work_bytes, need);
-  hipStream_t s = (hipStream_t)stream;
-  const int ik = (int)k;
-  if (type == FHIP_C64 && compute == FHIP_C32)
-    return launch_trsm<double, float, true>((double *)Y, ldy, m, ik, (const double *)R, ldr, (float *)work, s);
-  switch (type) {
-    case FHIP_C64:
-      return launch_trsm<double, double, true>((double *)Y, ldy, m, ik, (const double *)R, ldr, (double *)work, s);
-    case FHIP_C32:
-      return launch_trsm<float, float, true>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)work, s);
-    case FHIP_F64:
-      return launch_trsm<double, double, false>((double *)Y, ldy, m, ik, (const double *)R, ldr, (double *)work, s);
-    case FHIP_F32:
-      return launch_trsm<float, float, false>((float *)Y, ldy, m, ik, (const float *)R, ldr, (float *)work, s);
+  return trsm_runn_checked("fhip_trsm_runn", Y, ldy, m, k, R, ldr, type, compute, work, work_bytes,
+                           (hipStream_t)stream);
+}
+
+extern "C" int fhip_trsm_runn_noalloc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr,
+                                      int type, int compute, void *stream) {
+  if (valid_type(compute) && k >= 0 && k <= (1 << 16)) {
+    size_t need = 0;
+    fhip_trsm_workspace_bytes(&need, k, compute);
+    if (need > TRSM_SCRATCH_BYTES)
+      return fail(FHIP_E_ARGUMENT,
+                  "fhip_trsm_runn_noalloc: k = %lld needs a %zu-byte folded factor, past the %zu bytes of the "
+                  "library's fixed device scratch (k <= 1024 for complex FP64); use fhip_trsm_runn with a workspace",
+                  (long long)k, need, TRSM_SCRATCH_BYTES);
   }
-  return fail(FHIP_E_UNSUPPORTED, "fhip_trsm_runn: unknown type %d", type);
+  void *M = nullptr;
+  // Resolve the scratch only for a call that will launch: argument errors
+  // are reported without touching a device.
+  const bool launches = valid_type(type) && valid_type(compute) && k > 0 && k <= (1 << 16) && m >= 0 && Y && R &&
+                        ldr >= k && (m == 0 || ldy >= m);
+  if (launches) {
+    int st = scratch_address(&M);
+    if (st) return st;
+  }
+  return trsm_runn_checked("fhip_trsm_runn_noalloc", Y, ldy, m, k, R, ldr, type, compute, M,
+                           launches ? TRSM_SCRATCH_BYTES : 0, (hipStream_t)stream);
 }
 
 extern "C" int fhip_trsm_runc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr, int type,

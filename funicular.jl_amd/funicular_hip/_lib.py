@@ -82,6 +82,7 @@ _SIGNATURES = {
     "fhip_trsm_workspace_bytes": [ctypes.POINTER(_sz), _i64, _i],
     "fhip_trsm_runn": [_vp, _i64, _i64, _i64, _vp, _i64, _i, _i, _vp, _sz, _vp],
     "fhip_trsm_runc": [_vp, _i64, _i64, _i64, _vp, _i64, _i, _vp, _sz, _vp],
+    "fhip_trsm_runn_noalloc": [_vp, _i64, _i64, _i64, _vp, _i64, _i, _i, _vp],
     "fhip_gemm_nn": [_vp, _i64, _vp, _i64, _vp, _i64, _i64, _i64, _i64, _i, _vp],
     "fhip_convert_copy": [_vp, _i, _i64, _vp, _i, _i64, _i64, _i64, _vp],
     "fhip_fill": [_vp, _i, _i64, _i64, _i64, _d, _d, _vp],

@@ -182,6 +182,19 @@ int fhip_trsm_runn(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R,
                    int64_t ldr, int type, int compute, void *work, size_t work_bytes,
                    void *stream);
 
+/* fhip_trsm_runn without a workspace argument, and without allocating: the
+ * folded factor goes to a fixed 16 MiB device array inside the library, one
+ * per GPU, held from the first call on that GPU until the process ends.
+ * That bounds k to 1024 for FHIP_C64 compute (kp^2 x eltype bytes <= 16 MiB,
+ * kp = k rounded up to 64); a larger k fails with FHIP_E_ARGUMENT before any
+ * device work. The results are those of fhip_trsm_runn, bit for bit. Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (two
+ * streams, or two threads, without ordering between them) race. No
+ * counterpart in the reference: it is for callers that cannot allocate, such
+ * as code under stream capture or a serving loop that forbids hipMalloc. */
+int fhip_trsm_runn_noalloc(void *Y, int64_t ldy, int64_t m, int64_t k, const void *R, int64_t ldr,
+                           int type, int compute, void *stream);
+
 /* Y <- Y * R^-H in place (right side, R upper triangular k x k, conjugate
  * transpose): the solve of the QR adjoint, dY = (dQ + Q N) R^-H. Same
  * blocking as fhip_trsm_runn, walking the column blocks right to left. No

@@ -40,6 +40,14 @@ def test_argument_errors_need_no_device():
     assert lib.fhip_trsm_runn(None, 1, 1, 4, None, 4, 3, 1, None, 0, None) == _lib.FHIP_E_UNSUPPORTED
     assert lib.fhip_trsm_runn(None, 1, 1, 4, None, 4, 3, 2, None, 0, None) == _lib.FHIP_E_ARGUMENT
     assert lib.fhip_ghost_randn(None, 3, 10, 0, 10, 0, 1, 0, None) == _lib.FHIP_E_ARGUMENT
+    # The allocation-free solve refuses what its fixed scratch cannot hold,
+    # and bad arguments, before it looks for a device.
+    assert lib.fhip_trsm_runn_noalloc(None, 8, 8, 1025, None, 1025, 3, 3, None) == _lib.FHIP_E_ARGUMENT
+    assert b"scratch" in lib.fhip_last_error()
+    assert lib.fhip_trsm_runn_noalloc(None, 8, 8, 16, None, 16, 3, 3, None) == _lib.FHIP_E_ARGUMENT
+    assert b"null matrix" in lib.fhip_last_error()
+    assert lib.fhip_trsm_runn_noalloc(None, 8, 8, 16, None, 16, 3, 1, None) == _lib.FHIP_E_UNSUPPORTED
+    assert lib.fhip_trsm_runn_noalloc(None, 8, 8, 0, None, 16, 3, 3, None) == 0
     with pytest.raises(_lib.LibraryError) as e:
         _lib.call("fhip_convert_copy", None, 0, 1, None, 3, 1, 1, 1, None)
     assert "null pointer" in str(e.value)
