@@ -46,7 +46,7 @@ class DimensionMismatch(ValueError):
 
 
 def human_readable_bytes(n):
-    """src/device_api.jl:425-435."""
+    """src/device_api.jl:301-311."""
     n = int(n)
     if n < 1024:
         return f"{n} B"

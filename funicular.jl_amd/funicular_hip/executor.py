@@ -13,6 +13,7 @@ from the plan's pool and the host waits once, at the end.
 Device-resident operands (plans with device_resident=True) need no staging:
 their steps are views of the matrix's own HBM allocation.
 """
+import contextlib
 import queue as _queue
 import threading
 
@@ -170,6 +171,36 @@ def clear_pins(operands):
 
 _FAILED = object()
 
+# Observers of the sweeps started while trace_sweeps() is open: each gets one
+# record per sweep, before its first step is issued.
+_observers = []
+
+
+@contextlib.contextmanager
+def trace_sweeps():
+    """Collects a record of every sweep issued inside the block, in order:
+    the traversal ("rows", with each step's (start, stop) rows, or "panels",
+    with each step's (panel, label)), each operand's mode ("read", "write" or
+    "update") and whether it was staged, and the buffer count. The schedule
+    of an operation is then comparable step for step with the reference's."""
+    records = []
+    _observers.append(records.append)
+    try:
+        yield records
+    finally:
+        _observers.remove(records.append)
+
+
+def _trace(steps, operands, staged, nb):
+    if isinstance(steps, PanelSteps):
+        kind, seq = "panels", tuple(zip(steps.panels, steps.labels))
+    else:
+        kind, seq = "rows", tuple(steps.blocks)
+    modes = tuple("update" if op.read and op.write else "read" if op.read else "write" for op in operands)
+    record = {"traversal": kind, "steps": seq, "modes": modes, "staged": tuple(staged), "nbuffers": nb}
+    for observe in list(_observers):
+        observe(record)
+
 
 class StageChannel:
     """The producer half of a sweep (src/executor.jl:102-111): stages every
@@ -275,6 +306,8 @@ def sweep(f, *operands, nbuffers=None, traversal=None):
         return None
     nb = min(requested, count)
     staged = [not op.pm.resident for op in operands]
+    if _observers:
+        _trace(steps, operands, staged, nb)
     dims = [buffer_dims(steps, op.pm) for op in operands]
     buffers = []
     staging = []

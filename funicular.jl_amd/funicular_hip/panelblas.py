@@ -2,11 +2,14 @@
 elementwise scale / axpy / dot / norm (src/panelblas.jl:1-370), on the sweep
 pipeline and the HIP kernels.
 
-The step structure is the reference's: gram and the triangular solve sweep
-row blocks of height floor(N w / k); project runs p (p + 1) panel steps
-labelled (applying, i, j); cholqr2 is two passes, and a third when a shifted
-Cholesky fired. The k x k Cholesky factorizations run on the host in LAPACK,
-as in the reference (src/panelblas.jl:148-155).
+Under the default "reference" schedule the steps and copies are the
+reference's: gram and the triangular solve sweep row blocks of height
+floor(N w / k); cholqr2 is two passes, and a third when a shifted Cholesky
+fired, each pass a Gram sweep, the k x k Gram down to the host, its Cholesky
+in LAPACK (src/panelblas.jl:148-155), the factor up and a solve sweep; project
+runs p (p + 1) panel steps labelled (applying, i, j). The "fused" schedule
+(a plan's `schedule=` or the `schedule=` keyword) shortens them as the
+functions below describe; its results agree to rounding.
 """
 import numpy as np
 import scipy.linalg
@@ -17,6 +20,15 @@ from .executor import PanelSteps, RowSteps, readpanels, rowtraversal, sweep, upd
 from .operator import apply_operator, assert_operator_shape, assert_square_operator
 from .panels import (assert_conformal, assert_same_panels, assert_same_rows, host_storage_eltype, npanels, panelrange,
                      panelwidth, pooled_like)
+
+
+def resolve_schedule(pm, schedule=None):
+    """The schedule an operation on pm runs: the keyword when given, else the
+    plan's ("reference" unless the plan says otherwise)."""
+    s = pm.plan.schedule if schedule is None else schedule
+    if s not in _plan.SCHEDULES:
+        raise ArgumentError(f"schedule must be one of {', '.join(map(repr, _plan.SCHEDULES))}, got {s!r}")
+    return s
 
 
 def shared_plan(X, Y):
@@ -93,11 +105,18 @@ def panelmul(Y, G, X, *, nbuffers=None):
     return Y
 
 
-def cholqr2(Y, *, nbuffers=None):
+def cholqr2(Y, *, nbuffers=None, schedule=None):
     """Orthonormalizes Y in place by CholeskyQR2 and returns the k x k upper
     triangular R with Q R equal to the original Y (src/panelblas.jl:102-139).
     A pass whose Gram needed a shift triggers a third pass (shifted
-    CholeskyQR3, Fukaya et al.)."""
+    CholeskyQR3, Fukaya et al.).
+
+    schedule="reference" runs the reference's four row sweeps (six with the
+    third pass) with the Cholesky on the host; "fused" keeps G and R on the
+    device and, off HBM, forms each next pass's Gram inside the solve sweep,
+    so Y goes up three times instead of four."""
+    if resolve_schedule(Y, schedule) == "reference":
+        return _reference_passes(Y, nbuffers)
     return _passes(Y, nbuffers)
 
 
@@ -187,7 +206,7 @@ def _device_pass(Y, nbuffers, R_prev=None, first=None, gram_ready=None, fuse_nex
                 rounding = _plan.checkout_device_buffers(plan, narrow_to, (height, k), 1)[0]
         try:
             rdiv_rows(Y, R, nbuffers=nbuffers, compute=solve, factor=factor,
-                      next_gram=None if nxt is None else (nxt, work_g, rounding))
+                      next_gram=None if nxt is None else (nxt, work_g, rounding), schedule="fused")
         finally:
             total = R if product is None else product.result()
         handed, nxt = nxt, None
@@ -197,15 +216,45 @@ def _device_pass(Y, nbuffers, R_prev=None, first=None, gram_ready=None, fuse_nex
 
 
 def cholqr_pass_then(Y, R_prev, nbuffers=None):
-    """One more CholQR pass over Y, returning (R_pass R_prev, shifted). The
-    host forms the triangular product while the pass's solve sweep runs on the
-    device, instead of after it."""
+    """One more CholQR pass over Y on the fused schedule, returning
+    (R_pass R_prev, shifted). The host forms the triangular product while the
+    pass's solve sweep runs on the device, instead of after it."""
     return _device_pass(Y, nbuffers, R_prev=R_prev)[:2]
 
 
-def cholqr_pass(Y, nbuffers=None):
-    """src/panelblas.jl:141-146."""
+def cholqr_pass(Y, nbuffers=None, schedule=None):
+    """One CholQR pass, returning (R, shifted) (src/panelblas.jl:141-146):
+    the Gram sweep, the Cholesky (on the host under the reference schedule,
+    on the device under the fused one) and the solve sweep."""
+    if resolve_schedule(Y, schedule) == "reference":
+        G = gram(Y, nbuffers=nbuffers)
+        R, shifted = cholesky_upper(G, Y.N, Y.k)
+        rdiv_rows(Y, R, nbuffers=nbuffers, schedule="reference")
+        return R, shifted
     return _device_pass(Y, nbuffers)[:2]
+
+
+def _reference_passes(Y, nbuffers, first=None):
+    """cholqr2 (first=None) or cholqr2_mixed on the reference's schedule
+    (src/panelblas.jl:130-146): every pass is its own Gram sweep, host
+    Cholesky and solve sweep. first=(shift, solve) makes the first pass the
+    mixed one: its Gram on the FP32 matrix cores with FP64 sums, its
+    Cholesky in FP32 on the host, the solve in FP32 or FP64."""
+    if first is None:
+        R, shifted1 = cholqr_pass(Y, nbuffers, schedule="reference")
+    else:
+        shift, solve = first
+        G = gram(Y, nbuffers=nbuffers, compute=np.complex64)
+        R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
+        R = R32.astype(Y.dtype)
+        rdiv_rows(Y, R, nbuffers=nbuffers, compute=None if solve == "fp64" else np.complex64, schedule="reference")
+        shifted1 = s > 0.0
+    R2, shifted2 = cholqr_pass(Y, nbuffers, schedule="reference")
+    R = triangular_product(R2, R)
+    if shifted1 or shifted2:
+        R3, _ = cholqr_pass(Y, nbuffers, schedule="reference")
+        R = triangular_product(R3, R)
+    return R
 
 
 def _fusable(Y):
@@ -275,9 +324,12 @@ def cholesky_upper(G, N, k):
     return shifted, True
 
 
-def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None, next_gram=None):
-    """Y <- Y R^-1 over row blocks (src/panelblas.jl:157-169). `factor`, a
-    device copy of R already in place, saves the upload. next_gram=(acc,
+def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None, next_gram=None, schedule=None):
+    """Y <- Y R^-1 over row blocks (src/panelblas.jl:157-169), R uploaded
+    first. Under the fused schedule a matrix resident in HBM is solved in one
+    launch over all its rows instead of block by block. The keywords below
+    serve the fused cholqr2. `factor`, a device copy of R already in place,
+    saves the upload. next_gram=(acc,
     work) also accumulates the Gram of the solved blocks into the device
     matrix acc while each block is still on the device: the next CholQR
     pass's Gram, without a sweep of its own; next_gram=(acc, work, rounding)
@@ -314,7 +366,8 @@ def rdiv_rows(Y, R, *, nbuffers=None, compute=None, factor=None, next_gram=None)
         # blocks: one launch over all N rows leaves no partly filled last wave
         # of workgroups per block (3% of a 629,856-row launch). A Gram rides
         # along only on row blocks, its chunking being that of gram's sweep.
-        traversal = RowSteps(Y.N, Y.N) if (Y.resident and next_gram is None) else rowtraversal(Y)
+        whole = Y.resident and next_gram is None and resolve_schedule(Y, schedule) == "fused"
+        traversal = RowSteps(Y.N, Y.N) if whole else rowtraversal(Y)
         sweep(step, updatepanels(Y), nbuffers=nbuffers, traversal=traversal)
     finally:
         _plan.checkin_device_buffers(plan, [factor, work] if owned else [work])
@@ -405,8 +458,13 @@ def _pool_can_hold(plan, T, dims):
     return pool.allocated + int(dims[0]) * int(dims[1]) * np.dtype(T).itemsize <= pool.budget
 
 
-def project(Q, G, *, nbuffers=None, group=None):
+def project(Q, G, *, nbuffers=None, group=None, schedule=None):
     """S = Q^H G Q without storing G Q (src/panelblas.jl:247-301).
+
+    Under the reference schedule (the default) this is the reference's
+    p (p + 1) panel steps, labelled (applying, i, j), whatever tier Q is on;
+    `group` > 1 asks for runs of held images on any schedule. The fused
+    schedule picks its own route:
 
     A basis resident in HBM whose plan can lend a second N x k buffer takes a
     shorter route, since no panel has to travel: G Q is formed whole in that
@@ -423,6 +481,8 @@ def project(Q, G, *, nbuffers=None, group=None):
     as it is, its labels included."""
     plan = Q.plan
     assert_square_operator(G, Q.N)
+    if group is None and resolve_schedule(Q, schedule) == "reference":
+        group = 1
     if group is None and Q.resident and _pool_can_hold(plan, Q.dtype, (Q.N, Q.k)):
         with pooled_like(Q) as GQ:
             panelmul(GQ, G, Q, nbuffers=nbuffers)
@@ -782,7 +842,7 @@ def cholesky_upper_fp32(G, N, k, shift="adaptive"):
     return R, s
 
 
-def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp32"):
+def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp32", schedule=None):
     """The mixed first pass: FP32 matrix-core Gram with FP64 sums across row
     chunks and an FP32 Cholesky (shifted when needed), then the triangular
     solve on the FP64 data. solve="fp32" runs the solve's products on the FP32
@@ -796,20 +856,31 @@ def cholqr_pass_mixed(Y, nbuffers=None, shift="adaptive", solve="fp32"):
         raise ArgumentError(f"solve must be 'fp64' or 'fp32', got {solve!r}")
     if shift not in ("adaptive", "fukaya"):
         raise ArgumentError(f"shift must be 'adaptive' or 'fukaya', got {shift!r}")
+    if resolve_schedule(Y, schedule) == "reference":
+        G = gram(Y, nbuffers=nbuffers, compute=np.complex64)
+        R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
+        R = R32.astype(Y.dtype)
+        rdiv_rows(Y, R, nbuffers=nbuffers, compute=None if solve == "fp64" else np.complex64, schedule="reference")
+        return R, s
     return _device_pass(Y, nbuffers, first=(shift, solve))[:2]
 
 
-def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32"):
+def cholqr2_mixed(Y, *, nbuffers=None, shift="adaptive", solve="fp32", schedule=None):
     """cholqr2 with a mixed-precision first pass (FP32 Gram and Cholesky, the
     solve in the precision `solve` names) as a preconditioner and the second
     pass in FP64. A third FP64 pass runs when either earlier Cholesky needed a
     shift, as shifted CholeskyQR3 prescribes: a shifted first pass leaves Q1
     with a condition number well above 1, which one FP64 pass does not bring
-    to FP64 orthogonality."""
+    to FP64 orthogonality. The schedule is cholqr2's: under "reference" every
+    pass is a Gram sweep, a host Cholesky and a solve sweep (the first pass's
+    Cholesky in FP32); under "fused" the factorizations run on the device and
+    the Gram of each next pass rides on the solve sweep off HBM."""
     if Y.dtype != np.dtype(np.complex128):
         raise ArgumentError("the mixed-precision pass is for ComplexF64 panel matrices")
     if solve not in ("fp64", "fp32"):
         raise ArgumentError(f"solve must be 'fp64' or 'fp32', got {solve!r}")
     if shift not in ("adaptive", "fukaya"):
         raise ArgumentError(f"shift must be 'adaptive' or 'fukaya', got {shift!r}")
+    if resolve_schedule(Y, schedule) == "reference":
+        return _reference_passes(Y, nbuffers, first=(shift, solve))
     return _passes(Y, nbuffers, first=(shift, solve))

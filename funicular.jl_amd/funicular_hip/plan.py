@@ -28,6 +28,9 @@ def align_up(n, a):
     return cld(n, a) * a
 
 
+SCHEDULES = ("reference", "fused")
+
+
 class DevicePool:
     """src/plan.jl:9-15: buffers by (eltype, dims), reused after checkin."""
 
@@ -55,14 +58,25 @@ class ResidencyPlan:
 
     Keywords as in the reference: device_budget, host_budget, backend,
     scratch_dir, workspace_bytes, nbuffers, target_panel_bytes, panel_width,
-    host_eltype, disk_eltype. `device_resident=True` is this backend's
-    addition: panels of matrices built from the plan live in HBM for their
-    whole life, and sweeps over them skip the host tier (a DeviceTier).
+    host_eltype, disk_eltype. Two keywords are this backend's additions:
+
+    * `device_resident=True`: panels of matrices built from the plan live in
+      HBM for their whole life, and sweeps over them skip the host tier (a
+      DeviceTier).
+    * `schedule`: the default step and transfer schedule of the panel BLAS.
+      "reference" (the default) runs the reference's steps and copies one for
+      one: cholqr2 as two (or three) passes of a Gram row sweep, the k x k
+      Gram down to the host, a host Cholesky, the factor up and a solve row
+      sweep; project as its p (p + 1) panel steps. "fused" lets the backend
+      shorten them: the Cholesky on the device, the next pass's Gram formed
+      inside the solve sweep off HBM, one solve launch over all rows of a
+      resident matrix, and project by a resident G Q or by runs of held
+      images. The results agree to rounding; the transfers do not.
     """
 
     def __init__(self, *, device_budget, host_budget, backend=None, scratch_dir=None, workspace_bytes=0,
                  nbuffers=2, target_panel_bytes=2 * GiB, panel_width=None, host_eltype=None,
-                 disk_eltype=None, device_resident=False):
+                 disk_eltype=None, device_resident=False, schedule="reference"):
         if backend is None:
             from .device_api import hip_backend
             backend = hip_backend()
@@ -105,6 +119,9 @@ class ResidencyPlan:
         self.host_eltype = None if host_eltype is None else np.dtype(host_eltype)
         self.disk_eltype = None if disk_eltype is None else np.dtype(disk_eltype)
         self.device_resident = bool(device_resident)
+        if schedule not in SCHEDULES:
+            raise ArgumentError(f"schedule must be one of {', '.join(map(repr, SCHEDULES))}, got {schedule!r}")
+        self.schedule = schedule
         self.devicepool = DevicePool(dev - work)
         self.hostpool = HostPool(host)
 
@@ -115,6 +132,8 @@ class ResidencyPlan:
             s += f", scratch {self.scratch_dir!r}"
         if self.device_resident:
             s += ", device resident"
+        if self.schedule != "reference":
+            s += f", {self.schedule} schedule"
         return s + ")"
 
 

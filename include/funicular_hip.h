@@ -52,7 +52,7 @@ extern "C" {
 /* ---------------------------------------------------------------- errors -- */
 
 /* Message of the last failure on the calling thread ("" when none). Replaces
- * the exceptions the Julia backend throws (src/device_api.jl:415-423,
+ * the exceptions the Julia backend throws (src/device_api.jl:161,
  * ext/FunicularCUDAExt.jl:123). */
 const char *fhip_last_error(void);
 

@@ -61,7 +61,7 @@ def project_labels(p):
 
 
 def slot(step, nb):
-    """src/executor.jl:519: mod1(step, nb) for 1-based steps, here 0-based."""
+    """src/executor.jl:199: mod1(step, nb) for 1-based steps, here 0-based."""
     return step % nb
 
 

@@ -234,14 +234,16 @@ def test_device_resident_plan(backend, nb):
 
 
 @pytest.mark.parametrize("mixed", (False, True))
-def test_streamed_cholqr2_uploads_y_three_times(backend, mixed):
-    # Off HBM each solve sweep also forms the next pass's Gram, so cholqr2
-    # uploads Y three times (Gram, solve + Gram, solve) instead of four, and
-    # the bits match the resident run, whose passes are not fused.
+def test_fused_schedule_uploads_y_three_times(backend, mixed):
+    # On the fused schedule, off HBM, each solve sweep also forms the next
+    # pass's Gram, so cholqr2 uploads Y three times (Gram, solve + Gram,
+    # solve) instead of the reference's four, and the bits match the resident
+    # run, whose passes are not fused.
     rng = np.random.default_rng(501)
     N, k, w = 3000, 96, 16
     A = randmatrix(rng, N, k)
-    run = (lambda Y: F.cholqr2_mixed(Y)) if mixed else (lambda Y: F.cholqr2(Y))
+    run = ((lambda Y: F.cholqr2_mixed(Y, schedule="fused")) if mixed
+           else (lambda Y: F.cholqr2(Y, schedule="fused")))
     hp = make_plan(backend, device_budget=256 * 2 ** 20)
     Yh = F.PanelMatrix.from_array(A, plan=hp, w=w)
     c0 = F._lib.counters()
@@ -268,12 +270,12 @@ def test_fused_passes_round_to_a_narrower_host_tier(backend):
     plan = make_plan(backend, device_budget=256 * 2 ** 20, host_eltype=np.complex64)
     Y1 = F.PanelMatrix.from_array(A, plan=plan, w=w)
     c0 = F._lib.counters()
-    R1 = F.cholqr2(Y1)
+    R1 = F.cholqr2(Y1, schedule="fused")
     c1 = F._lib.counters()
     up = c1["h2d_bytes"] - c0["h2d_bytes"]
     assert 3 * N * k * 8 <= up < 3 * N * k * 8 + 8 * k * k * 16, up
     Y2 = F.PanelMatrix.from_array(A, plan=plan, w=w)
-    R2, s1 = cholqr_pass(Y2)
+    R2, s1 = cholqr_pass(Y2, schedule="fused")
     R2, s2 = cholqr_pass_then(Y2, R2)
     assert not (s1 or s2)
     assert np.array_equal(R1, R2)
@@ -282,8 +284,9 @@ def test_fused_passes_round_to_a_narrower_host_tier(backend):
 
 @pytest.mark.parametrize("w", (23, 8, 1))
 def test_project_on_a_device_resident_plan(backend, w):
-    # With room for a second N x k buffer, G Q is formed whole and S is one
-    # Gram sweep; with no such room, the reference's panel schedule runs.
+    # On the fused schedule, with room for a second N x k buffer, G Q is
+    # formed whole and S is one Gram sweep; with no such room, the
+    # reference's panel schedule runs.
     print(f"resident project w={w}", flush=True)
     rng = np.random.default_rng(700 + w)
     N, k = 20000, 64
@@ -298,7 +301,7 @@ def test_project_on_a_device_resident_plan(backend, w):
         plan = make_plan(backend, device_budget=budget, device_resident=True)
         G = F.Tridiagonal1D(N, d, o, backend=backend)
         Q = F.PanelMatrix.from_array(A, plan=plan, w=w)
-        S = F.project(Q, G)
+        S = F.project(Q, G, schedule="fused")
         assert OB.isapprox(S, ref, tol)
         assert np.array_equal(F.to_matrix(Q), A)
         # The pooled route leaves its N x k buffer in the pool for reuse.
@@ -426,7 +429,7 @@ def test_project_by_runs_of_held_images(backend, group):
     assert g == p or group is not None
     runs = -(-p // g)
     c0 = F._lib.counters()
-    S = F.project(Q, G, group=group)
+    S = F.project(Q, G, group=group, schedule="fused")
     c1 = F._lib.counters()
     assert OB.isapprox(S, ref, OB.blastol(C16, N, k))
     assert c1["h2d_bytes"] - c0["h2d_bytes"] == (1 + runs) * N * k * 16
@@ -447,7 +450,7 @@ def test_project_by_runs_over_a_disk_backed_basis(backend, tmp_path):
     Q = F.PanelMatrix.from_array(A, plan=plan, w=w)
     before = F.disk_reads(Q)
     for group in (2, None):
-        S = F.project(Q, G, group=group)
+        S = F.project(Q, G, group=group, schedule="fused")
         assert OB.isapprox(S, ref, OB.blastol(C16, N, k))
     assert F.disk_reads(Q) > before
     assert np.array_equal(F.to_matrix(Q), A)
@@ -507,8 +510,9 @@ def test_broadcast_closure_over_narrowed_storage(backend):
     assert np.array_equal(F.to_matrix(Z), 2 * A32)
 
 
+@pytest.mark.parametrize("schedule", ("reference", "fused"))
 @pytest.mark.parametrize("resident", (False, True))
-def test_no_device_allocation_after_warmup(backend, resident):
+def test_no_device_allocation_after_warmup(backend, resident, schedule):
     """test/test_cuda.jl:100-111 on the library's own counter: after one run to
     fill the pools, a data sweep, a copying sweep, panelmul, gram, cholqr2 and
     project allocate no device memory at all (every workspace and staging
@@ -516,7 +520,7 @@ def test_no_device_allocation_after_warmup(backend, resident):
     rng = np.random.default_rng(1000)
     N, k, w = 4000, 48, 8
     A = randmatrix(rng, N, k)
-    plan = make_plan(backend, device_budget=256 << 20, device_resident=resident)
+    plan = make_plan(backend, device_budget=256 << 20, device_resident=resident, schedule=schedule)
     G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, backend=backend)
     X = F.PanelMatrix.from_array(A, plan=plan, w=w)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
