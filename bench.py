@@ -139,29 +139,28 @@ def summarize(samples, flops_fn):
             "flops_per_launch": float(flops.mean()), "ms_per_row": float(ms.sum()) / rows}
 
 
-def cpu_baseline(k, w, rows, repeats=3):
+def cpu_baseline(k, w, rows, repeats=3, full_rows=10077696):
     """The oracle's cholqr2 (the reference CPU backend's algorithm: row-block
     Grams, host Cholesky, row-block triangular solves, FP64 LAPACK/BLAS) timed
     on this host's cores over a bounded sample of rows of the same workload:
     one untimed call, then the best of `repeats`, end to end and per operation
-    (gram, the Cholesky and rdiv_rows, as the reference's benchmarks report
-    them, benchmark/common.jl:12-15). project is timed once: its p (p + 1)
-    steps take several times a whole cholqr2 on the CPU."""
+    (panelmul!, gram, the Cholesky and rdiv_rows!, as the reference's
+    benchmarks report them, benchmark/common.jl:12-15). project is timed
+    once: its p (p + 1) steps take several times a whole cholqr2 on the CPU.
+    rows/s is the sample's rows over its time; the work per row is the same
+    at any N (the k x k Cholesky aside), so it stands for the full basis."""
     from oracle import ghost as OG
     from oracle import operators as OO
     from oracle import panelblas as OB
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max((i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"),
-                      default=os.cpu_count())
-    except Exception:
-        threads = os.cpu_count()
+    facts = host_facts()
+    threads = (facts.get("blas") or {}).get("threads") or os.cpu_count()
     n1 = 8
     n2 = 128
     n3 = rows // (n1 * n2)
     N = n1 * n2 * n3
     apply = lambda X: OO.helmholtz_apply(X, (n1, n2, n3), OO.helmholtz_diagonal())
-    Y0 = OB.panelmul(apply, OG.matrix(np.complex128, N, k, 0xDEADBEEF), w)
+    omega = OG.matrix(np.complex128, N, k, 0xDEADBEEF)
+    Y0 = OB.panelmul(apply, omega, w)
 
     def best_of(fn, n=repeats, warm=True):
         if warm:
@@ -178,15 +177,83 @@ def cpu_baseline(k, w, rows, repeats=3):
     G = OB.gram(Y0, w=w)
     R, _ = OB.cholesky_upper(G, N, k)
     Q, _ = OB.cholqr2(Y0, w)
-    per_op = {"gram_s": best_of(lambda: OB.gram(Y0, w=w)),
+    per_op = {"panelmul_s": best_of(lambda: OB.panelmul(apply, omega, w)),
+              "gram_s": best_of(lambda: OB.gram(Y0, w=w)),
               "cholesky_s": best_of(lambda: OB.cholesky_upper(G, N, k)),
               "rdiv_rows_s": best_of(lambda: OB.rdiv_rows(Y0, R, w)),
               "project_s": best_of(lambda: OB.project(Q, apply, w), n=1, warm=False)}
     return {"value": N / best, "unit": "rows/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle cholqr2 (numpy/LAPACK FP64, reference step order) of a {N}x{k} ComplexF64 "
-                      f"Helmholtz-applied ghost basis, panels of {w}, one untimed call then best of "
-                      f"{repeats}: {best:.2f} s",
+            "sample": f"oracle cholqr2 (numpy + OpenBLAS/LAPACK FP64, the reference's step order) of a {N}x{k} "
+                      f"ComplexF64 Helmholtz-applied ghost basis in panels of {w}: {N} of the {full_rows} rows "
+                      f"({100 * N / full_rows:.1f}%), one untimed call then best of {repeats}: {best:.2f} s, "
+                      f"so {N / best:.0f} rows/s; {threads} BLAS threads",
+            "sample_rows": N, "sample_s": best, "host": facts,
             "per_operation": {key: round(v, 4) for key, v in per_op.items()}}
+
+
+def host_facts():
+    """The host the CPU baseline runs on: CPU model, core counts, memory,
+    the page-lock limit of this process, and the BLAS numpy calls."""
+    import platform
+    import resource
+    import subprocess
+    facts = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+             "machine": platform.machine()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                facts[key.strip().lower().replace(" ", "_").replace("(s)", "s")] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal:"):
+                facts["mem_total_GiB"] = round(int(line.split()[1]) / 2 ** 20, 1)
+    except OSError:
+        pass
+    soft, hard = resource.getrlimit(resource.RLIMIT_MEMLOCK)
+    facts["memlock_limit"] = "unlimited" if soft == resource.RLIM_INFINITY else soft
+    try:
+        from threadpoolctl import threadpool_info
+        blas = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        if blas:
+            b = blas[0]
+            facts["blas"] = {"library": b.get("internal_api"), "version": b.get("version"),
+                             "threads": b.get("num_threads"), "architecture": b.get("architecture"),
+                             "file": os.path.basename(b.get("filepath", ""))}
+    except Exception:
+        pass
+    return facts
+
+
+def pinned_probe(limit_gib=160, chunk_gib=8):
+    """How much host memory this process can page-lock: fhip_host_alloc in
+    chunk_gib pieces until a call fails or limit_gib is reached, then every
+    piece freed. Prints one JSON line with the host's facts."""
+    import ctypes
+    import funicular_hip as F
+    from funicular_hip import _lib
+    F.hip_backend()
+    lib = _lib.load()
+    held, error = [], None
+    t = time.perf_counter()
+    while len(held) * chunk_gib < limit_gib:
+        p = ctypes.c_void_p()
+        st = lib.fhip_host_alloc(ctypes.byref(p), chunk_gib << 30)
+        if st != 0:
+            error = lib.fhip_last_error().decode(errors="replace")
+            break
+        held.append(p.value)
+        log(f"pinned-probe: {len(held) * chunk_gib} GiB page-locked")
+    dt = time.perf_counter() - t
+    locked = len(held) * chunk_gib
+    for p in held:
+        lib.fhip_host_free(p)
+    print(json.dumps({"page_locked_GiB": locked, "limit_tried_GiB": limit_gib, "chunk_GiB": chunk_gib,
+                      "stopped_by": "error" if error else "limit", "error": error,
+                      "seconds": round(dt, 1), "host": host_facts()}), flush=True)
 
 
 def residual(backend, Y, R, G, omega):
@@ -355,14 +422,21 @@ def mixed_bench(backend, steps):
         F.free(Z)
         return float(np.sqrt(num / np.real(np.trace(F.gram(Y0)))))
 
+    from funicular_hip.executor import trace_sweeps
     for decades in (2, 4):
         sigma = 10.0 ** (-decades * np.arange(k) / (k - 1))
         F.rightmul(Y0, U, sigma[:, None] * V.conj().T)
         case = {}
+        # The configuration's stated shift, 11 (N k + k (k + 1)) u32 ||Y||_2^2,
+        # always applied ("fukaya"), next to the adaptive default; a shifted
+        # first pass makes a third (FP64) pass run.
         for name, fn in (("fp64", F.cholqr2), ("mixed", cholqr2_mixed),
-                         ("mixed_fp64_solve", lambda Z: cholqr2_mixed(Z, solve="fp64"))):
+                         ("mixed_fp64_solve", lambda Z: cholqr2_mixed(Z, solve="fp64")),
+                         ("mixed_fukaya", lambda Z: cholqr2_mixed(Z, shift="fukaya")),
+                         ("mixed_fukaya_fp64_solve", lambda Z: cholqr2_mixed(Z, shift="fukaya", solve="fp64"))):
             copy(Y, Y0)
-            R = fn(Y)  # accuracy from the first run
+            with trace_sweeps() as sweeps:
+                R = fn(Y)  # accuracy from the first run
             orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
             res = resid(R)
             times = []
@@ -373,9 +447,11 @@ def mixed_bench(backend, steps):
                 fn(Y)
                 backend.sync_all()
                 times.append(time.perf_counter() - t)
-            case[name] = {"s": min(times), "orthogonality": orth, "residual": res}
-        case["speedup"] = case["fp64"]["s"] / case["mixed"]["s"]
-        case["speedup_fp64_solve"] = case["fp64"]["s"] / case["mixed_fp64_solve"]["s"]
+            case[name] = {"s": min(times), "orthogonality": orth, "residual": res, "passes": len(sweeps) // 2}
+        for name in ("mixed", "mixed_fp64_solve", "mixed_fukaya", "mixed_fukaya_fp64_solve"):
+            case[name]["speedup"] = case["fp64"]["s"] / case[name]["s"]
+        case["speedup"] = case["mixed"]["speedup"]
+        case["speedup_fp64_solve"] = case["mixed_fp64_solve"]["speedup"]
         out[f"sigma_to_1e-{decades}"] = case
         log(f"mixed workload, sigma to 1e-{decades}: {case}")
     for pm in (U, Y0, Y):
@@ -624,28 +700,171 @@ def kernels_only(rows, k, reps, vendor=False):
     print(json.dumps(out), flush=True)
 
 
-def streamed(args, phase_gap=0.0):
-    """The reference's own regime: the basis lives in page-locked host memory
-    (host_budget 128 GiB), every sweep streams row blocks up (and, for the
-    solve, back down) through nbuffers = 2 staging buffers in a 64 GiB device
-    budget, copies on their own streams fenced by events. Reports the
-    PCIe-inclusive cholqr2 rate, the host-link GB/s, and the fraction of
-    transfer hidden behind compute for a Gram sweep,
-    (t_copy + t_compute - t_wall) / min(t_copy, t_compute). phase_gap > 0
-    leaves the GPU idle that many seconds between the phases (panelmul, the
-    two copy-only sweeps, the Gram sweep, cholqr2, project by runs, project
-    on the reference schedule, the final Gram)."""
+class CopyTimer:
+    """HIP timing events around every staging copy a sweep issues (uploads
+    on the sweep's upload stream, write-backs on its download stream), so a
+    streamed operation's copies can be laid against its kernels on one time
+    axis."""
+
+    def __init__(self):
+        import ctypes
+        from funicular_hip import _lib
+        from funicular_hip import executor
+        self.ctypes = ctypes
+        self._lib = _lib
+        self.executor = executor
+        self.records = []
+        self.active = False
+        self._orig_in = executor.stage_in
+        self._orig_out = executor.stage_out
+        timer = self
+
+        def stage_in(t, dst, pm, step, queue, via=None):
+            ev = timer._record(queue)
+            r = timer._orig_in(t, dst, pm, step, queue, via=via)
+            timer._done(ev, queue, "up")
+            return r
+
+        def stage_out(t, pm, src, step, queue, via=None):
+            ev = timer._record(queue)
+            r = timer._orig_out(t, pm, src, step, queue, via=via)
+            timer._done(ev, queue, "down")
+            return r
+        executor.stage_in = stage_in
+        executor.stage_out = stage_out
+
+    def _event(self, stream):
+        e = self.ctypes.c_void_p()
+        self._lib.call("fhip_event_create", self.ctypes.byref(e), 0)
+        self._lib.call("fhip_event_record", e, stream)
+        return e.value
+
+    def _record(self, queue):
+        if not self.active or queue is None:
+            return None
+        return self._event(queue.stream)
+
+    def _done(self, start, queue, kind):
+        if start is not None:
+            self.records.append((start, self._event(queue.stream), kind))
+
+
+def _intervals(lib, ctypes, origin, records):
+    """[(start_ms, stop_ms)] of event pairs, measured from `origin`."""
+    out = []
+    for start, stop in records:
+        lib.call("fhip_event_sync", stop)
+        a, b = ctypes.c_float(), ctypes.c_float()
+        lib.call("fhip_event_elapsed", ctypes.byref(a), origin, start)
+        lib.call("fhip_event_elapsed", ctypes.byref(b), origin, stop)
+        out.append((a.value, b.value))
+        lib.call("fhip_event_destroy", start)
+        lib.call("fhip_event_destroy", stop)
+    return out
+
+
+def _union(iv):
+    merged = []
+    for a, b in sorted(iv):
+        if merged and a <= merged[-1][1]:
+            merged[-1][1] = max(merged[-1][1], b)
+        else:
+            merged.append([a, b])
+    return merged
+
+
+def _length(merged):
+    return sum(b - a for a, b in merged)
+
+
+def _overlap(m1, m2):
+    total, i, j = 0.0, 0, 0
+    while i < len(m1) and j < len(m2):
+        lo, hi = max(m1[i][0], m2[j][0]), min(m1[i][1], m2[j][1])
+        total += max(0.0, hi - lo)
+        if m1[i][1] < m2[j][1]:
+            i += 1
+        else:
+            j += 1
+    return total
+
+
+def overlapped(backend, kernels, copies, fn):
+    """Runs fn with the kernel and copy timers on and returns (wall seconds,
+    result, timeline) where the timeline gives, from the HIP events on the
+    streams the work ran on, the time the kernels (Gram and solve) and the
+    copies occupied, and how much of each the other covered: hidden_copy is
+    the fraction of copy time that ran under kernels, hidden_compute the
+    fraction of kernel time that ran under copies. Both are fractions of
+    measured intervals, so they lie in [0, 1]."""
+    import ctypes
+    from funicular_hip import _lib
+    backend.sync_all()
+    origin = ctypes.c_void_p()
+    _lib.call("fhip_event_create", ctypes.byref(origin), 0)
+    _lib.call("fhip_event_record", origin, None)
+    kernels.active = copies.active = True
+    t = time.perf_counter()
+    out = fn()
+    backend.sync_all()
+    wall = time.perf_counter() - t
+    kernels.active = copies.active = False
+    kiv = []
+    for kind in ("gram", "trsm"):
+        kiv += [(a, b) for a, b, _ in kernels.records[kind]]
+    kernels.records = {"gram": [], "trsm": []}
+    k_iv = _union(_intervals(_lib, ctypes, origin.value, kiv))
+    up = _union(_intervals(_lib, ctypes, origin.value, [(a, b) for a, b, kind in copies.records if kind == "up"]))
+    down = _union(_intervals(_lib, ctypes, origin.value, [(a, b) for a, b, kind in copies.records if kind == "down"]))
+    copies.records = []
+    c_iv = _union([tuple(x) for x in up + down])
+    _lib.call("fhip_event_destroy", origin.value)
+    t_k, t_c = _length(k_iv) / 1e3, _length(c_iv) / 1e3
+    both = _overlap(k_iv, c_iv) / 1e3
+    return wall, out, {"kernel_s": t_k, "copy_s": t_c, "upload_s": _length(up) / 1e3,
+                       "download_s": _length(down) / 1e3, "overlap_s": both,
+                       "hidden_compute": both / t_k if t_k > 0 else None,
+                       "hidden_copy": both / t_c if t_c > 0 else None,
+                       "link_busy_fraction": t_c / wall if wall > 0 else None}
+
+
+STREAMED_GRIDS = (200, 180)
+
+
+def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False):
+    """Configuration 2's own regime: FP64 host storage, the basis in
+    page-locked host memory, a 64 GiB device budget, nbuffers = 2, every
+    sweep streaming row blocks up (and, for the solve, back down) with the
+    copies on their own streams fenced by events. The boxes page-lock at most
+    64 GiB of host memory, below the 216^3 x 512 basis (82.6 GB), so the run
+    takes the largest grid of `grids` whose basis page-locks under a 62 GiB
+    host budget. Reports, for the reference schedule and the fused one, the
+    PCIe-inclusive cholqr2 rate, the bytes and rates over the link, and the
+    kernel/copy timeline (HIP events on the sweep's streams); for a Gram sweep
+    also (t_copy + t_compute - t_wall) / min(t_copy, t_compute) with t_copy a
+    copy-only sweep at the same nbuffers. phase_gap > 0 leaves the GPU idle
+    that many seconds between phases, so a trace can be cut at the gaps."""
+    import gc
     import funicular_hip as F
     from funicular_hip.executor import readpanels, rowtraversal, sweep
     backend = F.hip_backend()
-    # The machines this was measured on cap page-locked host memory at 64 GiB,
-    # below the 216^3 x 512 basis (82.6 GB); the streamed run takes the largest
-    # grid whose basis fits (180^3 rows: 47.8 GB) under a 60 GiB host budget.
-    n = args.stream_grid
-    N, k, w = n ** 3, args.k, args.w
-    plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=60 * 1024 ** 3)
-    t = time.perf_counter()
-    Y = F.PanelMatrix(N, k, plan=plan, w=w)
+    k, w = args.k, args.w
+    tried = []
+    plan = Y = None
+    for n in grids:
+        N = n ** 3
+        plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=62 * 1024 ** 3)
+        t = time.perf_counter()
+        try:
+            Y = F.PanelMatrix(N, k, plan=plan, w=w)
+            break
+        except Exception as err:  # page-locking refused: try the next grid
+            tried.append({"grid": n, "error": str(err)[:200]})
+            log(f"streamed: a {n}^3 x {k} basis did not page-lock ({err}); trying a smaller grid")
+            Y = plan = None
+            gc.collect()
+    if Y is None:
+        return {"error": "no grid page-locked", "tried": tried}
     log(f"streamed: {N}x{k} host-resident basis allocated and page-locked in {time.perf_counter() - t:.1f} s")
     d = 6.0 - 0.09 * (1 + 0.05j)
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
@@ -653,9 +872,11 @@ def streamed(args, phase_gap=0.0):
     c0 = F._lib.counters()
     t = time.perf_counter()
     F.panelmul(Y, G, omega)
+    backend.sync_all()
     t_panelmul = time.perf_counter() - t
     c1 = F._lib.counters()
-    log(f"streamed: panelmul {t_panelmul:.2f} s, {(c1['d2h_bytes'] - c0['d2h_bytes']) / t_panelmul / 1e9:.1f} GB/s down")
+    down_pm = c1["d2h_bytes"] - c0["d2h_bytes"]
+    log(f"streamed: panelmul {t_panelmul:.2f} s, {down_pm / t_panelmul / 1e9:.1f} GB/s down")
     traversal = rowtraversal(Y)
 
     def timed(fn):
@@ -672,36 +893,60 @@ def streamed(args, phase_gap=0.0):
 
     noop = lambda *a: None
     t_copy1, up1, _, _ = timed(lambda: sweep(noop, readpanels(Y), nbuffers=1, traversal=traversal))
-    t_copy2, _, _, _ = timed(lambda: sweep(noop, readpanels(Y), nbuffers=2, traversal=traversal))
-    timer = KernelTimer(backend)
-    timer.active = True
-    t_gram, _, _, _ = timed(lambda: F.gram(Y))
-    timer.active = False
-    t_compute = sum(ms for ms, _, _ in timer.collect()[("gram", False)]) / 1e3
-    hidden = (t_copy1 + t_compute - t_gram) / min(t_copy1, t_compute)
-    log(f"streamed: copy-only sweep nb=1 {t_copy1:.2f} s ({up1 / t_copy1 / 1e9:.1f} GB/s), nb=2 {t_copy2:.2f} s; "
-        f"gram sweep {t_gram:.2f} s with {t_compute:.3f} s of kernels; hidden {hidden:.3f}")
-    t_chol, up, down, R = timed(lambda: F.cholqr2(Y))
-    log(f"streamed: cholqr2 {t_chol:.2f} s, {N / t_chol:.0f} rows/s, up {up / 1e9:.1f} GB, down {down / 1e9:.1f} GB")
-    # project on the orthonormal basis: by runs of held panel images (the
-    # default off HBM), then the reference's p (p + 1)-step schedule.
-    t_proj, up_proj, _, S = timed(lambda: F.project(Y, G))
-    t_ref, up_ref, _, S_ref = timed(lambda: F.project(Y, G, group=1))
-    agree = float(np.linalg.norm(S - S_ref) / np.linalg.norm(S_ref))
-    log(f"streamed: project {t_proj:.2f} s ({up_proj / 1e9:.0f} GB up) against {t_ref:.2f} s "
-        f"({up_ref / 1e9:.0f} GB up) on the reference schedule; ||S - S_ref|| / ||S_ref|| = {agree:.1e}")
+    t_copy2, up2, _, _ = timed(lambda: sweep(noop, readpanels(Y), nbuffers=2, traversal=traversal))
+    kernels = KernelTimer(backend)
+    copies = CopyTimer()
     if phase_gap > 0:
         time.sleep(phase_gap)
-    orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
-    return {"cholqr2_s": t_chol, "rows_per_s_pcie_inclusive": N / t_chol, "h2d_GB": up / 1e9,
-            "d2h_GB": down / 1e9, "link_GBps_during_cholqr2": (up + down) / t_chol / 1e9,
-            "copy_only_sweep_nb1_s": t_copy1, "copy_only_sweep_nb2_s": t_copy2,
-            "h2d_GBps_copy_only": up1 / t_copy1 / 1e9, "gram_sweep_s": t_gram, "gram_kernels_s": t_compute,
-            "hidden_fraction": hidden, "panelmul_s": t_panelmul, "orthogonality": orth,
-            "project_s": t_proj, "project_h2d_GB": up_proj / 1e9, "project_reference_schedule_s": t_ref,
-            "project_reference_schedule_h2d_GB": up_ref / 1e9, "project_relative_difference": agree,
-            "project_check": project_check(S, d),
-            "N": N, "budgets": "device 64 GiB, host 60 GiB page-locked, nbuffers 2"}
+    t_gram, _, gram_tl = overlapped(backend, kernels, copies, lambda: F.gram(Y))
+    t_compute = gram_tl["kernel_s"]
+    formula = (t_copy2 + t_compute - t_gram) / min(t_copy2, t_compute)
+    log(f"streamed: copy-only sweep nb=1 {t_copy1:.2f} s ({up1 / t_copy1 / 1e9:.1f} GB/s), nb=2 {t_copy2:.2f} s "
+        f"({up2 / t_copy2 / 1e9:.1f} GB/s); gram sweep {t_gram:.2f} s with {t_compute:.3f} s of kernels, "
+        f"{gram_tl['hidden_compute']:.3f} of them under copies")
+    out = {"grid": n, "N": N, "k": k, "panel_width": w, "basis_GB": N * k * 16 / 1e9,
+           "budgets": "device 64 GiB, host 62 GiB page-locked (the box's limit is 64 GiB), FP64 host storage, "
+                      "nbuffers 2",
+           "grids_refused": tried, "panelmul_s": t_panelmul, "d2h_GBps_panelmul": down_pm / t_panelmul / 1e9,
+           "copy_only_sweep_nb1_s": t_copy1, "copy_only_sweep_nb2_s": t_copy2,
+           "h2d_GBps_copy_only_nb1": up1 / t_copy1 / 1e9, "h2d_GBps_copy_only_nb2": up2 / t_copy2 / 1e9,
+           "gram_sweep": dict(gram_tl, wall_s=t_gram,
+                              overlap_formula=formula,
+                              overlap_formula_definition="(t_copy + t_compute - t_wall) / min(t_copy, t_compute), "
+                                                         "t_copy = copy-only sweep at nbuffers 2")}
+    for schedule in ("reference", "fused"):
+        if phase_gap > 0:
+            time.sleep(phase_gap)
+        c0 = F._lib.counters()
+        t_chol, R, tl = overlapped(backend, kernels, copies, lambda: F.cholqr2(Y, schedule=schedule))
+        c1 = F._lib.counters()
+        up, down = c1["h2d_bytes"] - c0["h2d_bytes"], c1["d2h_bytes"] - c0["d2h_bytes"]
+        orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+        out[f"cholqr2_{schedule}"] = dict(tl, s=t_chol, rows_per_s_pcie_inclusive=N / t_chol,
+                                          h2d_GB=up / 1e9, d2h_GB=down / 1e9,
+                                          h2d_GBps=up / t_chol / 1e9, d2h_GBps=down / t_chol / 1e9,
+                                          link_GBps=(up + down) / t_chol / 1e9, orthogonality=orth)
+        log(f"streamed: cholqr2 ({schedule} schedule) {t_chol:.2f} s, {N / t_chol:.0f} rows/s, "
+            f"up {up / 1e9:.1f} GB, down {down / 1e9:.1f} GB, link busy {tl['link_busy_fraction']:.3f}, "
+            f"kernels under copies {tl['hidden_compute']:.3f}, ||Q^H Q - I||_F {orth:.1e}")
+    # project on the orthonormal basis: by runs of held panel images (fused),
+    # and on request the reference's p (p + 1)-step schedule, which uploads the
+    # basis p + 1 times.
+    t_proj, up_proj, _, S = timed(lambda: F.project(Y, G, schedule="fused"))
+    out["project_fused"] = {"s": t_proj, "h2d_GB": up_proj / 1e9, "check": project_check(S, d)}
+    log(f"streamed: project (fused, runs of held images) {t_proj:.2f} s, {up_proj / 1e9:.0f} GB up")
+    if project_reference:
+        t_ref, up_ref, _, S_ref = timed(lambda: F.project(Y, G, schedule="reference"))
+        out["project_reference"] = {"s": t_ref, "h2d_GB": up_ref / 1e9,
+                                    "relative_difference": float(np.linalg.norm(S - S_ref) / np.linalg.norm(S_ref))}
+        log(f"streamed: project (reference schedule) {t_ref:.2f} s, {up_ref / 1e9:.0f} GB up")
+    # Put the sweep functions back for whatever runs next.
+    import funicular_hip.executor as ex
+    ex.stage_in, ex.stage_out = copies._orig_in, copies._orig_out
+    backend.rdiv_upper, backend.rdiv_upper_folded = kernels._orig_trsm, kernels._orig_folded
+    backend.gram_accumulate = kernels._orig_gram
+    F.free(Y)
+    return out
 
 
 def streamed_narrow(args):
@@ -739,7 +984,7 @@ def streamed_narrow(args):
     log(f"streamed-narrow: copy-only sweep {t_copy:.2f} s, {h2d_copy:.1f} GB/s up")
     c0 = F._lib.counters()
     t = time.perf_counter()
-    F.cholqr2(Y)
+    F.cholqr2(Y, schedule="fused")
     backend.sync_all()
     t_chol = time.perf_counter() - t
     c1 = F._lib.counters()
@@ -751,7 +996,7 @@ def streamed_narrow(args):
     g = F.panelblas.project_group(Y)
     c0 = F._lib.counters()
     t = time.perf_counter()
-    S = F.project(Y, G)
+    S = F.project(Y, G, schedule="fused")
     backend.sync_all()
     t_proj = time.perf_counter() - t
     up_proj = F._lib.counters()["h2d_bytes"] - c0["h2d_bytes"]
@@ -762,6 +1007,7 @@ def streamed_narrow(args):
             "project_s": t_proj,
             "project_h2d_GB": up_proj / 1e9, "project_run_length": g, "project_check": project_check(S, d),
             "N": N,
+            "schedule": "fused (three uploads of Y per cholqr2; project by runs of held images)",
             "budgets": "device 64 GiB, host 60 GiB page-locked, host_eltype ComplexF32, nbuffers 2"}
 
 
@@ -776,35 +1022,41 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload (N = 2^22, k = 128)")
+    ap.add_argument("--no-streamed", action="store_true",
+                    help="skip the host-resident (streamed over PCIe) cholqr2 of the largest grid that page-locks")
+    ap.add_argument("--streamed-project-reference", action="store_true",
+                    help="in the streamed run, also time project on the reference schedule (p + 1 uploads)")
     ap.add_argument("--disk", action="store_true", help="also measure the disk tier (17 GB scratch file)")
     ap.add_argument("--range-finder", action="store_true",
                     help="also run the disk-tier range finder with power iterations (34 GB of scratch files)")
     ap.add_argument("--cpu-rows", type=int, default=1 << 17)
-    ap.add_argument("--streamed", action="store_true",
-                    help="also run the host-resident (streamed over PCIe) cholqr2 and report overlap")
-    ap.add_argument("--stream-grid", type=int, default=180, help="grid edge of the streamed run")
     ap.add_argument("--streamed-narrow", action="store_true",
                     help="also run the full-size streamed cholqr2 with ComplexF32 host storage")
     ap.add_argument("--streamed-only", action="store_true",
                     help="profile mode: only the streamed run, its phases separated by 0.25 s of GPU idle so "
                          "a trace can be cut at the gaps")
+    ap.add_argument("--pinned-probe", action="store_true",
+                    help="only report how much host memory can be page-locked (up to 160 GiB) and the host's facts")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profile mode: only the per-launch kernels at the row-block shape")
     ap.add_argument("--vendor", action="store_true",
                     help="with --kernels-only: also time rocBLAS zherk / ztrsm / zgemm on the same shapes")
     args = ap.parse_args()
-    if args.kernels_only:
-        kernels_only(args.grid ** 3 * args.w // args.k, args.k, max(args.steps, 1), vendor=args.vendor)
-        return
-    if args.streamed_only:
-        print(json.dumps(streamed(args, phase_gap=0.25)), flush=True)
-        return
-
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the path shards by rows with one all-reduce of the k x k Gram; see the design notes).",
               file=sys.stderr)
         sys.exit(2)
+    if args.pinned_probe:
+        pinned_probe()
+        return
+    if args.kernels_only:
+        kernels_only(args.grid ** 3 * args.w // args.k, args.k, max(args.steps, 1), vendor=args.vendor)
+        return
+    if args.streamed_only:
+        print(json.dumps(streamed(args, phase_gap=0.25, project_reference=args.streamed_project_reference)),
+              flush=True)
+        return
 
     import funicular_hip as F
     from funicular_hip.panelblas import cholqr2_mixed
@@ -812,6 +1064,7 @@ def main():
     backend = F.hip_backend()
     n = args.grid
     N, k, w = n ** 3, args.k, args.w
+    extra_steps = max(1, min(args.steps, 5))
     log(f"bench: N={N} k={k} w={w} on {backend.arch}")
     plan = F.ResidencyPlan(backend=backend, device_budget=240 * 1024 ** 3, host_budget=1 << 30,
                            device_resident=True)
@@ -831,9 +1084,10 @@ def main():
     log(f"bench: panelmul {t_panelmul:.3f} s")
     timer = KernelTimer(backend)
 
-    def run(mode, steps, warmup):
-        fn = {"fp64": F.cholqr2, "mixed": cholqr2_mixed,
-              "mixed-fp64-solve": lambda Y: cholqr2_mixed(Y, solve="fp64")}[mode]
+    def run(mode, schedule, steps, warmup):
+        fn = {"fp64": lambda Z: F.cholqr2(Z, schedule=schedule),
+              "mixed": lambda Z: cholqr2_mixed(Z, schedule=schedule),
+              "mixed-fp64-solve": lambda Z: cholqr2_mixed(Z, solve="fp64", schedule=schedule)}[mode]
         regenerate()
         first = fn(Y)  # from the real basis: accuracy
         backend.sync_all()
@@ -849,52 +1103,82 @@ def main():
         backend.sync_all()
         dt = time.perf_counter() - t
         timer.active = False
-        log(f"bench: {mode} {steps} x cholqr2 in {dt:.3f} s, ||Q^H Q - I||_F = {orth:.2e}, "
+        log(f"bench: {mode} ({schedule} schedule) {steps} x cholqr2 in {dt:.3f} s, ||Q^H Q - I||_F = {orth:.2e}, "
             f"||Y - QR||/||Y|| = {resid:.2e}")
         return dt, orth, first, timer.collect(), resid
 
-    dt64, orth64, R64, k64, res64 = run("fp64", args.steps, args.warmup)
+    # The headline: the reference schedule (every pass a Gram row sweep, the
+    # Gram down, a host Cholesky, R up, a solve row sweep) on a basis held in
+    # HBM.
+    dt64, orth64, R64, k64, res64 = run("fp64", "reference", args.steps, args.warmup)
     ms_step = dt64 / args.steps * 1e3
     value = N * args.steps / dt64
     gram64 = summarize(k64[("gram", False)], gram_flops)
     trsm64 = summarize(k64[("trsm", False)], trsm_flops)
+    h = F.row_block_height(N, k, w)
+    kernels = {
+        "gram": {"kernel": "gram_partial_kernel + gram_reduce_kernel (FP64)", "achieved": gram64["tflops"],
+                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": gram64["tflops"] / FP64_PEAK_TFLOPS,
+                 "traffic": measured_traffic("gram_fp64") if (N, k, w) == (10077696, 512, 32) else None,
+                 "avg_launch_ms": gram64["avg_ms"], "flops_per_launch": gram64["flops_per_launch"],
+                 "algorithmic_bytes_per_launch": 16 * h * k, "launches": gram64["launches"],
+                 "total_ms_per_step": gram64["avg_ms"] * gram64["launches"] / args.steps},
+        "trsm": {"kernel": "trsm_wide_kernel (FP64; on a sweep's first block also the two fold kernels)",
+                 "achieved": trsm64["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": trsm64["tflops"] / FP64_PEAK_TFLOPS,
+                 "traffic": measured_traffic("trsm_fp64") if (N, k, w) == (10077696, 512, 32) else None,
+                 "avg_launch_ms": trsm64["avg_ms"], "flops_per_launch": trsm64["flops_per_launch"],
+                 "algorithmic_bytes_per_launch": 32 * h * k, "launches": trsm64["launches"],
+                 "total_ms_per_step": trsm64["avg_ms"] * trsm64["launches"] / args.steps},
+    }
+    dominant = max(kernels, key=lambda name: kernels[name]["total_ms_per_step"])
     record = {
         "metric": METRIC, "value": value, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"cholqr2 of a device-resident {N}x{k} ComplexF64 basis: {n}^3 periodic "
-                               f"Helmholtz stencil applied to GhostPanels(seed 0xdeadbeef), panels of {w}",
-                   "N": N, "k": k, "panel_width": w, "row_block": F.row_block_height(N, k, w),
+        "config": {"workload": f"cholqr2 of a {N}x{k} ComplexF64 basis held in HBM (device-resident plan): "
+                               f"{n}^3 periodic Helmholtz stencil applied to GhostPanels(seed 0xdeadbeef), "
+                               f"panels of {w}, on the reference schedule (4 row sweeps of {h}-row blocks, "
+                               "host Cholesky)",
+                   "N": N, "k": k, "panel_width": w, "row_block": h, "schedule": "reference",
+                   "tier": "device-resident (the streamed host-resident run is the 'streamed' record)",
                    "element": "ComplexF64 (FP64 arithmetic on real and imaginary parts)"},
-        "roofline": {"bound": "mfma", "kernel": "gram_partial_kernel + gram_reduce_kernel (FP64)",
-                     "achieved": gram64["tflops"], "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": gram64["tflops"] / FP64_PEAK_TFLOPS,
-                     "traffic": measured_traffic("gram_fp64") if (N, k, w) == (10077696, 512, 32) else None,
-                     "avg_launch_ms": gram64["avg_ms"], "flops_per_launch": gram64["flops_per_launch"]},
-        "trsm": {"kernel": "trsm_wide_kernel (FP64; the fold kernels once per sweep)", "achieved": trsm64["tflops"],
-                 "peak": FP64_PEAK_TFLOPS, "frac": trsm64["tflops"] / FP64_PEAK_TFLOPS,
-                 "avg_launch_ms": trsm64["avg_ms"]},
+        "roofline": dict({"bound": "mfma"}, **{key: kernels[dominant][key] for key in
+                                                ("kernel", "achieved", "peak", "unit", "frac", "traffic",
+                                                 "avg_launch_ms", "flops_per_launch",
+                                                 "algorithmic_bytes_per_launch")}),
+        "kernels": kernels,
         "errors": {"fp64_orthogonality": orth64, "fp64_residual": res64},
         "panelmul_s": t_panelmul,
     }
-    # project on the orthonormal basis cholqr2 left in Y.
-    F.project(Y, G)
-    backend.sync_all()
-    t = time.perf_counter()
-    S = F.project(Y, G)
-    backend.sync_all()
-    t_project = time.perf_counter() - t
-    record["project"] = {"s": t_project, "reduction_tflops": 8.0 * N * k * k / t_project / 1e12,
-                         "antihermitian_check": project_check(S, d)}
-    log(f"bench: project {t_project:.3f} s, S - S^H check {record['project']['antihermitian_check']:.2e}")
+    # The same on the fused schedule (device Cholesky, one solve launch over
+    # all rows), an opt-in.
+    dtf, orthf64, _, kf, resf64 = run("fp64", "fused", extra_steps, 1)
+    record["fused_schedule"] = {"rows_per_s": N * extra_steps / dtf, "ms_per_step": dtf / extra_steps * 1e3,
+                                "steps": extra_steps, "orthogonality": orthf64, "residual": resf64,
+                                "speedup_vs_reference_schedule": (dt64 / args.steps) / (dtf / extra_steps),
+                                "trsm_tflops": summarize(kf[("trsm", False)], trsm_flops)["tflops"]}
+    # project on the orthonormal basis cholqr2 left in Y: the reference's
+    # p (p + 1) steps, and the fused resident route (G Q whole, one Gram sweep).
+    for schedule in ("reference", "fused"):
+        F.project(Y, G, schedule=schedule)
+        backend.sync_all()
+        t = time.perf_counter()
+        S = F.project(Y, G, schedule=schedule)
+        backend.sync_all()
+        t_project = time.perf_counter() - t
+        record[f"project_{schedule}"] = {"s": t_project, "reduction_tflops": 8.0 * N * k * k / t_project / 1e12,
+                                         "antihermitian_check": project_check(S, d)}
+        log(f"bench: project ({schedule}) {t_project:.3f} s, S - S^H check "
+            f"{record[f'project_{schedule}']['antihermitian_check']:.2e}")
     if not args.no_mixed:
-        dts, orths, Rs, ks, ress = run("mixed-fp64-solve", args.steps, args.warmup)
-        dtm, orthm, Rm, km, resm = run("mixed", args.steps, args.warmup)
+        dts, orths, Rs, ks, ress = run("mixed-fp64-solve", "reference", args.steps, args.warmup)
+        dtm, orthm, Rm, km, resm = run("mixed", "reference", args.steps, args.warmup)
         regenerate()
         Rf = cholqr2_mixed(Y, shift="fukaya")
-        orthf = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
-        resf = residual(backend, Y, Rf, G, omega)
-        log(f"bench: mixed with the Fukaya shift: ||Q^H Q - I||_F = {orthf:.2e}, ||Y - QR||/||Y|| = {resf:.2e}")
+        orthfk = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+        resfk = residual(backend, Y, Rf, G, omega)
+        log(f"bench: mixed with the Fukaya shift: ||Q^H Q - I||_F = {orthfk:.2e}, ||Y - QR||/||Y|| = {resfk:.2e}")
         g32 = summarize(km[("gram", True)], gram_flops)
         t32 = summarize(km[("trsm", True)], trsm_flops)
         # Per-pass ratio: the Gram and solve kernels of one pass over all N rows
@@ -902,18 +1186,30 @@ def main():
         # live launch timings.
         pass64 = N * (gram64["ms_per_row"] + trsm64["ms_per_row"])
         pass32 = N * (g32["ms_per_row"] + t32["ms_per_row"])
-        record["mixed"] = {"rows_per_s": N * args.steps / dtm, "ms_per_step": dtm / args.steps * 1e3,
-                           "speedup_vs_fp64": dt64 / dtm, "orthogonality": orthm, "residual": resm,
-                           "first_pass": "FP32 Gram (FP64 chunk sums), FP32 Cholesky, FP32 solve",
-                           "shift": "adaptive (unshifted FP32 Cholesky unless it fails)",
-                           "fukaya_shift_orthogonality": orthf, "fukaya_shift_residual": resf,
-                           "gram_fp32_tflops": g32["tflops"], "trsm_fp32_tflops": t32["tflops"],
-                           "first_pass_vs_fp64_pass": {"fp64_pass_kernels_ms": pass64,
-                                                       "fp32_pass_kernels_ms": pass32,
-                                                       "speedup": pass64 / pass32},
-                           "R_rel_diff_vs_fp64": float(np.linalg.norm(Rm - R64) / np.linalg.norm(R64)),
-                           "fp64_solve": {"rows_per_s": N * args.steps / dts, "speedup_vs_fp64": dt64 / dts,
-                                          "orthogonality": orths, "residual": ress}}
+        dtmf, orthmf, _, _, resmf = run("mixed", "fused", extra_steps, 1)
+        dtsf, orthsf, _, _, ressf = run("mixed-fp64-solve", "fused", extra_steps, 1)
+        step_f = dtf / extra_steps
+        record["mixed"] = {
+            "schedule": "reference (each pass a Gram sweep, host Cholesky, solve sweep), against the FP64 "
+                        "reference-schedule headline",
+            "rows_per_s": N * args.steps / dtm, "ms_per_step": dtm / args.steps * 1e3,
+            "speedup_vs_fp64": dt64 / dtm, "orthogonality": orthm, "residual": resm,
+            "speedup_at_orthogonality_bound": dt64 / dtm,
+            "speedup_at_both_bounds": dt64 / dts,
+            "first_pass": "FP32 Gram (FP64 chunk sums), FP32 Cholesky, FP32 solve",
+            "shift": "adaptive (unshifted FP32 Cholesky unless it fails)",
+            "fukaya_shift_orthogonality": orthfk, "fukaya_shift_residual": resfk,
+            "gram_fp32_tflops": g32["tflops"], "trsm_fp32_tflops": t32["tflops"],
+            "gram_fp32_frac_of_fp32_peak": g32["tflops"] / FP32_PEAK_TFLOPS,
+            "trsm_fp32_frac_of_fp32_peak": t32["tflops"] / FP32_PEAK_TFLOPS,
+            "first_pass_vs_fp64_pass": {"fp64_pass_kernels_ms": pass64, "fp32_pass_kernels_ms": pass32,
+                                        "speedup": pass64 / pass32},
+            "R_rel_diff_vs_fp64": float(np.linalg.norm(Rm - R64) / np.linalg.norm(R64)),
+            "fp64_solve": {"rows_per_s": N * args.steps / dts, "speedup_vs_fp64": dt64 / dts,
+                           "orthogonality": orths, "residual": ress},
+            "fused": {"speedup_vs_fused_fp64": step_f / (dtmf / extra_steps), "orthogonality": orthmf,
+                      "residual": resmf, "fp64_solve_speedup_vs_fused_fp64": step_f / (dtsf / extra_steps),
+                      "fp64_solve_orthogonality": orthsf, "fp64_solve_residual": ressf}}
     # The 216^3 x 512 basis is done with: release it before the other
     # workloads allocate theirs.
     del S
@@ -933,12 +1229,10 @@ def main():
         import tempfile
         with tempfile.TemporaryDirectory(prefix="funicular-rf-") as d:
             record["range_finder"] = range_finder_bench(backend, d)
-    if args.streamed or args.streamed_narrow:
-        del Y, omega, plan
-        import gc
-        gc.collect()
-    if args.streamed:
-        record["streamed"] = streamed(args)
+    del Y, omega, plan
+    gc.collect()
+    if not args.no_streamed:
+        record["streamed"] = streamed(args, project_reference=args.streamed_project_reference)
         gc.collect()
     if args.streamed_narrow:
         record["streamed_narrow"] = streamed_narrow(args)
