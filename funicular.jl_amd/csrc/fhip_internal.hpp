@@ -197,7 +197,7 @@ __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__rest
                                     int mi0, const T *__restrict__ Bs_re, const T *__restrict__ Bs_im,
                                     int ldb_s, int nj0, int s_begin, int s_end, int s_stride,
                                     typename Mfma<T>::acc (&p1)[MB][NB], typename Mfma<T>::acc (&p2)[MB][NB],
-                                    typename Mfma<T>::acc (&p3)[MB][NB]) {
+                                    typename Mfma<T>::acc (&p3)[MB][NB], unsigned nb_mask = ~0u) {
   const int lane = threadIdx.x & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
@@ -218,6 +218,7 @@ __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__rest
     }
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
+      if (!((nb_mask >> nb) & 1u)) continue;
       br[nb] = Bs_re[kr * ldb_s + ((nj0 + 16 * nb + li) ^ sw)];
       bi[nb] = Bs_im[kr * ldb_s + ((nj0 + 16 * nb + li) ^ sw)];
       bt[nb] = (MODE == PROD_CONJ_A) ? br[nb] - bi[nb] : br[nb] + bi[nb];
@@ -226,6 +227,9 @@ __device__ inline void mma_steps_3m(const T *__restrict__ As_re, const T *__rest
     for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
       for (int nb = 0; nb < NB; ++nb) {
+        // nb_mask (wave-uniform) leaves out column tiles whose slice of B is
+        // known to be zero.
+        if (!((nb_mask >> nb) & 1u)) continue;
         p1[mb][nb] = Mfma<T>::mma(ar[mb], br[nb], p1[mb][nb]);
         p2[mb][nb] = Mfma<T>::mma(ai[mb], bi[nb], p2[mb][nb]);
         p3[mb][nb] = Mfma<T>::mma(as[mb], bt[nb], p3[mb][nb]);

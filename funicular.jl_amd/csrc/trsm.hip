@@ -11,8 +11,8 @@
 //     X_J = [Y_J, X_0, ..., X_{J-1}] * [M_JJ; M_0J; ...; M_{J-1,J}],
 // so the whole solve is a chain of sums that never leaves the matrix cores'
 // pipeline: the workgroup streams 16-column slices of its rows and of M
-// through LDS, the next slice's loads in flight during the current slice's
-// three-product (3M) complex steps, across block boundaries too. Each block
+// through LDS, the loads of the slice after next in flight during the current
+// slice's three-product (3M) complex steps, across block boundaries too. Each block
 // reads its unsolved columns first and the solved ones after, so a slice in
 // flight never races the block being written: the newest solved block is
 // read several barriers after its stores. M is formed once per call in
@@ -228,25 +228,39 @@ __global__ __launch_bounds__(256) void trsm_offdiag_kernel(const TS *__restrict_
 // solves X R^H = Y, whose factor is lower triangular (blocks right to left).
 // Each wave computes a 32 x 32 piece of the workgroup's TW_ROWS x TW_CB block
 // as 2 x 2 matrix-core tiles.
+//
+// Thread t stages rows t % TW_ROWS of columns t / TW_ROWS + YCS e of each Y
+// slice and rows t % KC of columns t / KC + MCS e of each M slice, so the
+// address of every element it loads is its base plus a wave-uniform offset
+// (kc and the element index times the column stride): one 64-bit add per
+// load, no 64-bit multiply and no per-element branch. The slice schedule is
+// kept in scalars updated once per slice.
+//
+// trsm_wide_short_kernel issues each slice's loads one slice ahead; it fits
+// the registers of two waves per SIMD for both directions, and serves R^H.
+// trsm_wide_kernel (below) issues them two slices ahead.
 template <typename TS, typename T, bool CPLX, bool CT>
-__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void trsm_wide_kernel(
+__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void trsm_wide_short_kernel(
     TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int TW_KC = TrsmSlice<T>::KC;
   constexpr int TW_APT = TrsmSlice<T>::APT;
   constexpr int TW_BPT = TrsmSlice<T>::BPT;
-  // Complex planes use a swizzled LDS layout with unpadded rows: the 8-byte
-  // swizzle for FP64, the 4-byte one for FP32 (whose planes must be 64 wide
-  // with 16-row slices); the others pad each row by 4.
   constexpr int SWZ = !(CPLX && FHIP_TRSM_SWIZZLE) ? 0
                       : sizeof(T) == 8                                     ? 1
                       : (TW_ROWS == 64 && TW_CB == 64 && TW_KC == 16)      ? 2
                                                                            : 0;
   constexpr int TW_LDA = SWZ ? TW_ROWS : TW_ROWS + 4;
   constexpr int TW_LDB = SWZ ? TW_CB : TW_CB + 4;
+  constexpr int YCS = TW_THREADS / TW_ROWS;  // column step between a thread's Y elements
+  constexpr int MCS = TW_THREADS / TW_KC;    // column step between a thread's M elements
+  static_assert(TW_THREADS % TW_ROWS == 0 && TW_THREADS % TW_KC == 0, "staging layout");
+  static_assert(YCS * TW_APT == TW_KC && MCS * TW_BPT == TW_CB, "staging covers the slice");
   __shared__ T As[NP][TW_KC * TW_LDA];
   __shared__ T Bs[NP][TW_KC * TW_LDB];
   typedef typename Mfma<T>::acc acc_t;
+  typedef typename Pair<TS>::type PS;
+  typedef typename Pair<T>::type PT;
 
   const int64_t row0 = (int64_t)blockIdx.x * TW_ROWS;
   const int t = threadIdx.x;
@@ -254,86 +268,71 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
   const int lane = t & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
-  constexpr int WAVE_ROWS = TW_ROWS / 32;  // waves down the rows of the block
+  constexpr int WAVE_ROWS = TW_ROWS / 32;
   const int mi0 = 32 * (w % WAVE_ROWS);
   const int nj0 = 32 * (w / WAVE_ROWS);
   const int nbk = (k + TW_CB - 1) / TW_CB;
 
-  // Block of step s and the slices of a block: its own columns first, then
-  // the solved ones (left of it for R, right of it for R^H).
-  auto block_of = [&](int s) { return CT ? nbk - 1 - s : s; };
-  auto own_slices = [&](int J) {
-    const int c0 = J * TW_CB;
-    const int nb = (k - c0) < TW_CB ? (k - c0) : TW_CB;
-    return (nb + TW_KC - 1) / TW_KC;
-  };
-  auto nslices = [&](int J) {
-    const int c0 = J * TW_CB;
-    const int solved = CT ? k - (c0 + TW_CB < k ? c0 + TW_CB : k) : c0;
-    return own_slices(J) + (solved + TW_KC - 1) / TW_KC;
-  };
-  auto slice_col = [&](int J, int idx) {
-    const int c0 = J * TW_CB;
-    const int own = own_slices(J);
-    if (idx < own) return c0 + TW_KC * idx;
-    return CT ? c0 + TW_CB + TW_KC * (idx - own) : TW_KC * (idx - own);
-  };
+  // Per-thread bases. A row past m reads row 0 (its results are never
+  // stored); a column past k is zeroed when staged.
+  const int yr = t % TW_ROWS, yk = t / TW_ROWS;
+  const int mk = t % TW_KC, mc = t / TW_KC;
+  const int64_t row = row0 + yr;
+  const bool row_ok = row < m;
+  const TS *ybase = Y + (int64_t)NP * ((row_ok ? row : 0) + (int64_t)yk * ldy);
+  const T *mbase = M + (int64_t)NP * (mk + (int64_t)mc * kp);
+  const int64_t ystride = (int64_t)NP * ldy;  // one column of Y
+  const int64_t mstride = (int64_t)NP * kp;   // one column of M
 
-  // Staging registers. Unconverted data is loaded straight into the compute
-  // registers; converted data (FP32 products of FP64 storage) is held as the
-  // raw stored pairs and converted only when staged: a conversion next to a
-  // guarded load makes the compiler wait for each load on the spot.
-  constexpr bool CONVERTED = !std::is_same<TS, T>::value;
-  T ar[TW_APT], ai[TW_APT], bre[TW_BPT], bim[TW_BPT];
-  typename Pair<TS>::type ya[TW_APT];
+  PS ya[TW_APT];
+  PT ma[TW_BPT];
   bool oka[TW_APT];
-  // Slice at column kc for block J: columns [kc, kc + 16) of the
-  // workgroup's rows of Y and rows [kc, kc + 16) x block J's columns of M,
-  // zero past k.
-  auto fetch = [&](int J, int kc) {
-    const int c0 = J * TW_CB;
+  // Slice at column kc of block J (first column c0): columns [kc, kc + KC)
+  // of the workgroup's rows of Y and rows [kc, kc + KC) x the block's
+  // columns of M. Every M load stays inside the kp x kp factor, whose padding
+  // past k is zero.
+  auto fetch = [&](int c0, int kc) {
+    const TS *yp = ybase + (int64_t)kc * ystride;
 #pragma unroll
     for (int e = 0; e < TW_APT; ++e) {
-      const int idx = t + TW_THREADS * e;
-      const int r = idx % TW_ROWS;
-      const int kk = idx / TW_ROWS;
-      const int64_t row = row0 + r;
-      const bool ok = row < m && kc + kk < k;
-      if constexpr (CONVERTED) {
-        oka[e] = ok;
-        ya[e] = load_raw<TS, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, ok);
+      const bool ok = row_ok && kc + yk + YCS * e < k;
+      oka[e] = ok;
+      const TS *p = ok ? yp + (int64_t)(YCS * e) * ystride : Y;
+      if constexpr (CPLX) {
+        ya[e] = *reinterpret_cast<const PS *>(p);
       } else {
-        load_elem<TS, T, CPLX>(Y, row + (int64_t)(kc + kk) * ldy, ok, ar[e], ai[e]);
+        ya[e].x = *p;
+        ya[e].y = TS(0);
       }
     }
+    const T *mp = mbase + (int64_t)NP * kc + (int64_t)c0 * mstride;
 #pragma unroll
     for (int e = 0; e < TW_BPT; ++e) {
-      const int idx = t + TW_THREADS * e;
-      const int kk = idx % TW_KC;
-      const int c = idx / TW_KC;
-      load_elem<T, T, CPLX>(M, (int64_t)(kc + kk) + (int64_t)(c0 + c) * kp, kc + kk < k && c0 + c < k, bre[e],
-                            bim[e]);
+      const T *p = mp + (int64_t)(MCS * e) * mstride;
+      if constexpr (CPLX) {
+        ma[e] = *reinterpret_cast<const PT *>(p);
+      } else {
+        ma[e].x = *p;
+        ma[e].y = T(0);
+      }
     }
   };
   auto stash = [&]() {
 #pragma unroll
     for (int e = 0; e < TW_APT; ++e) {
-      const int idx = t + TW_THREADS * e;
-      const int r = idx % TW_ROWS;
-      const int kk = idx / TW_ROWS;
-      const int at = kk * TW_LDA + (r ^ swizzle_of<SWZ>(kk));
-      if constexpr (CONVERTED) from_raw(ya[e], oka[e], ar[e], ai[e]);
-      As[0][at] = ar[e];
-      if (CPLX) As[NP - 1][at] = ai[e];
+      const int kk = yk + YCS * e;
+      const int at = kk * TW_LDA + (yr ^ swizzle_of<SWZ>(kk));
+      T re, im;
+      from_raw(ya[e], oka[e], re, im);
+      As[0][at] = re;
+      if (CPLX) As[NP - 1][at] = im;
     }
 #pragma unroll
     for (int e = 0; e < TW_BPT; ++e) {
-      const int idx = t + TW_THREADS * e;
-      const int kk = idx % TW_KC;
-      const int c = idx / TW_KC;
-      const int at = kk * TW_LDB + (c ^ swizzle_of<SWZ>(kk));
-      Bs[0][at] = bre[e];
-      if (CPLX) Bs[NP - 1][at] = bim[e];
+      const int c = mc + MCS * e;
+      const int at = mk * TW_LDB + (c ^ swizzle_of<SWZ>(mk));
+      Bs[0][at] = ma[e].x;
+      if (CPLX) Bs[NP - 1][at] = ma[e].y;
     }
   };
 
@@ -347,28 +346,34 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
       p3[mb][nb] = Mfma<T>::zero();
     }
 
+  // The slice schedule: block J = the step's block, its own slices first
+  // (from c0), then the solved ones (from 0 for R, from c0 + CB for R^H).
+  auto own_of = [&](int c0) { return ((k - c0 < TW_CB ? k - c0 : TW_CB) + TW_KC - 1) / TW_KC; };
+  auto solved_start = [&](int c0) { return CT ? c0 + TW_CB : 0; };
+  auto count_of = [&](int c0) {
+    const int solved = CT ? k - (c0 + TW_CB < k ? c0 + TW_CB : k) : c0;
+    return own_of(c0) + (solved + TW_KC - 1) / TW_KC;
+  };
   int step = 0;
-  int J = block_of(0);
+  int c0 = (CT ? nbk - 1 : 0) * TW_CB;
+  int own = own_of(c0), n = count_of(c0);
   int idx = 0;
-  int n = nslices(J);
-  fetch(J, slice_col(J, 0));
+  int kc = c0;
+  fetch(c0, kc);
   while (true) {
-    const int kc = slice_col(J, idx);
     __syncthreads();
     stash();
     __syncthreads();
     // Prefetch the next slice, crossing into the next block.
-    {
-      const bool same = idx + 1 < n;
-      const int Jn = same ? J : block_of(step + 1);
-      if (same || step + 1 < nbk) fetch(Jn, slice_col(Jn, same ? idx + 1 : 0));
+    if (idx + 1 < n) {
+      const int kcn = idx + 1 < own ? kc + TW_KC : (idx + 1 == own ? solved_start(c0) : kc + TW_KC);
+      fetch(c0, kcn);
+    } else if (step + 1 < nbk) {
+      const int c0n = (CT ? nbk - 2 - step : step + 1) * TW_CB;
+      fetch(c0n, c0n);
     }
-    const int c0 = J * TW_CB;
-    // Waves whose columns lie past k only stage slices. Within the diagonal
-    // block, M_JJ is triangular: slice rows past the wave's last column add
-    // nothing (before its first column for the lower-triangular M_JJ of R^H).
     bool contributes = c0 + nj0 < k;
-    if (kc >= c0 && kc < c0 + TW_CB) {
+    if (idx < own) {
       const int kl = kc - c0;
       contributes = contributes && (CT ? (kl + TW_KC > nj0) : (kl < nj0 + 32));
     }
@@ -380,7 +385,10 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
                                             1, p1, p2);
     }
-    if (++idx < n) continue;
+    if (++idx < n) {
+      kc = idx < own ? kc + TW_KC : (idx == own ? solved_start(c0) : kc + TW_KC);
+      continue;
+    }
     // Block J is solved. Every wave has staged its last slice of Y_J (the
     // barriers above), so the stores cannot race a load of the unsolved
     // columns; the next blocks read X_J only after further barriers.
@@ -392,9 +400,9 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         if constexpr (CPLX) combine3m<PROD_ADD>(p1[mb][nb], p2[mb][nb], p3[mb][nb], re, im);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int64_t row = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
+          const int64_t orow = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
           const int col = c0 + nj0 + 16 * nb + li;
-          if (row < m && col < k) store_elem<TS, CPLX>(Y, row + (int64_t)col * ldy, (TS)re[r], (TS)im[r]);
+          if (orow < m && col < k) store_elem<TS, CPLX>(Y, orow + (int64_t)col * ldy, (TS)re[r], (TS)im[r]);
         }
       }
     if (++step == nbk) break;
@@ -406,9 +414,224 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         p2[mb][nb] = Mfma<T>::zero();
         p3[mb][nb] = Mfma<T>::zero();
       }
-    J = block_of(step);
+    c0 = (CT ? nbk - 1 - step : step) * TW_CB;
+    own = own_of(c0);
+    n = count_of(c0);
     idx = 0;
-    n = nslices(J);
+    kc = c0;
+  }
+}
+
+// The solve with each slice's global loads issued two slices ahead: two sets
+// of staging registers alternate, so a slice's loads have two slices of
+// products to land in instead of one (the reads of solved columns come back
+// from HBM). The loop body is unrolled twice so that each set is a fixed set
+// of registers. Within a diagonal block, a 16-row slice of the triangular
+// M_JJ meets each 16-column tile of a wave as a zero block, a triangle or a
+// full block; the products with zero blocks are left out.
+template <typename TS, typename T, bool CPLX, bool CT>
+__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void trsm_wide_kernel(
+    TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int TW_KC = TrsmSlice<T>::KC;
+  constexpr int TW_APT = TrsmSlice<T>::APT;
+  constexpr int TW_BPT = TrsmSlice<T>::BPT;
+  constexpr int SWZ = !(CPLX && FHIP_TRSM_SWIZZLE) ? 0
+                      : sizeof(T) == 8                                     ? 1
+                      : (TW_ROWS == 64 && TW_CB == 64 && TW_KC == 16)      ? 2
+                                                                           : 0;
+  constexpr int TW_LDA = SWZ ? TW_ROWS : TW_ROWS + 4;
+  constexpr int TW_LDB = SWZ ? TW_CB : TW_CB + 4;
+  constexpr int YCS = TW_THREADS / TW_ROWS;
+  constexpr int MCS = TW_THREADS / TW_KC;
+  static_assert(TW_THREADS % TW_ROWS == 0 && TW_THREADS % TW_KC == 0, "staging layout");
+  static_assert(YCS * TW_APT == TW_KC && MCS * TW_BPT == TW_CB, "staging covers the slice");
+  __shared__ T As[NP][TW_KC * TW_LDA];
+  __shared__ T Bs[NP][TW_KC * TW_LDB];
+  typedef typename Mfma<T>::acc acc_t;
+  typedef typename Pair<TS>::type PS;
+  typedef typename Pair<T>::type PT;
+
+  const int64_t row0 = (int64_t)blockIdx.x * TW_ROWS;
+  const int t = threadIdx.x;
+  const int w = t >> 6;
+  const int lane = t & 63;
+  const int li = lane & 15;
+  const int lk = lane >> 4;
+  constexpr int WAVE_ROWS = TW_ROWS / 32;
+  const int mi0 = 32 * (w % WAVE_ROWS);
+  const int nj0 = 32 * (w / WAVE_ROWS);
+  const int nbk = (k + TW_CB - 1) / TW_CB;
+
+  const int yr = t % TW_ROWS, yk = t / TW_ROWS;
+  const int mk = t % TW_KC, mc = t / TW_KC;
+  const int64_t row = row0 + yr;
+  const bool row_ok = row < m;
+  const TS *ybase = Y + (int64_t)NP * ((row_ok ? row : 0) + (int64_t)yk * ldy);
+  const T *mbase = M + (int64_t)NP * (mk + (int64_t)mc * kp);
+  const int64_t ystride = (int64_t)NP * ldy;
+  const int64_t mstride = (int64_t)NP * kp;
+
+  PS ya[2][TW_APT];
+  PT ma[2][TW_BPT];
+  bool oka[2][TW_APT];
+  auto fetch = [&](auto set, int c0, int kc) {
+    constexpr int S = decltype(set)::value;
+    const TS *yp = ybase + (int64_t)kc * ystride;
+#pragma unroll
+    for (int e = 0; e < TW_APT; ++e) {
+      const bool ok = row_ok && kc + yk + YCS * e < k;
+      oka[S][e] = ok;
+      const TS *p = ok ? yp + (int64_t)(YCS * e) * ystride : Y;
+      if constexpr (CPLX) {
+        ya[S][e] = *reinterpret_cast<const PS *>(p);
+      } else {
+        ya[S][e].x = *p;
+        ya[S][e].y = TS(0);
+      }
+    }
+    const T *mp = mbase + (int64_t)NP * kc + (int64_t)c0 * mstride;
+#pragma unroll
+    for (int e = 0; e < TW_BPT; ++e) {
+      const T *p = mp + (int64_t)(MCS * e) * mstride;
+      if constexpr (CPLX) {
+        ma[S][e] = *reinterpret_cast<const PT *>(p);
+      } else {
+        ma[S][e].x = *p;
+        ma[S][e].y = T(0);
+      }
+    }
+  };
+  auto stash = [&](auto set) {
+    constexpr int S = decltype(set)::value;
+#pragma unroll
+    for (int e = 0; e < TW_APT; ++e) {
+      const int kk = yk + YCS * e;
+      const int at = kk * TW_LDA + (yr ^ swizzle_of<SWZ>(kk));
+      T re, im;
+      from_raw(ya[S][e], oka[S][e], re, im);
+      As[0][at] = re;
+      if (CPLX) As[NP - 1][at] = im;
+    }
+#pragma unroll
+    for (int e = 0; e < TW_BPT; ++e) {
+      const int c = mc + MCS * e;
+      const int at = mk * TW_LDB + (c ^ swizzle_of<SWZ>(mk));
+      Bs[0][at] = ma[S][e].x;
+      if (CPLX) Bs[NP - 1][at] = ma[S][e].y;
+    }
+  };
+
+  acc_t p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      p1[mb][nb] = Mfma<T>::zero();
+      p2[mb][nb] = Mfma<T>::zero();
+      p3[mb][nb] = Mfma<T>::zero();
+    }
+
+  // Slice iterators: the slice being multiplied and the next one to fetch.
+  // A block's own slices come first. When a block has solved slices after a
+  // single slice of its own, a second, empty own slice is staged (columns
+  // past k load as zeros; no products are formed from it): the first solved
+  // slice is then fetched two slices after the previous block was stored,
+  // never before.
+  struct It {
+    int step, c0, real, own, n, idx, kc;
+    bool valid;
+  };
+  auto start = [&](It &it, int step) {
+    it.step = step;
+    it.c0 = (CT ? nbk - 1 - step : step) * TW_CB;
+    it.real = ((k - it.c0 < TW_CB ? k - it.c0 : TW_CB) + TW_KC - 1) / TW_KC;
+    const int solved = CT ? k - (it.c0 + TW_CB < k ? it.c0 + TW_CB : k) : it.c0;
+    const int nsolved = (solved + TW_KC - 1) / TW_KC;
+    it.own = (it.real < 2 && nsolved > 0) ? 2 : it.real;
+    it.n = it.own + nsolved;
+    it.idx = 0;
+    it.kc = it.c0;
+    it.valid = step < nbk;
+  };
+  auto next = [&](It &it) {
+    if (++it.idx < it.n) {
+      it.kc = it.idx < it.own ? it.kc + TW_KC : (it.idx == it.own ? (CT ? it.c0 + TW_CB : 0) : it.kc + TW_KC);
+    } else {
+      start(it, it.step + 1);
+    }
+  };
+  It cur, pf;
+  start(cur, 0);
+  pf = cur;
+  fetch(std::integral_constant<int, 0>(), pf.c0, pf.kc);
+  next(pf);
+  if (pf.valid) {
+    fetch(std::integral_constant<int, 1>(), pf.c0, pf.kc);
+    next(pf);
+  }
+  // One slice: stage set S, refill it two slices ahead, multiply, and store
+  // a finished block. Returns false after the last slice.
+  auto slice = [&](auto set) {
+    __syncthreads();
+    stash(set);
+    __syncthreads();
+    if (pf.valid) {
+      fetch(set, pf.c0, pf.kc);
+      next(pf);
+    }
+    const bool last = cur.idx + 1 >= cur.n;
+    const int c0 = cur.c0;
+    bool contributes = c0 + nj0 < k && !(cur.idx >= cur.real && cur.idx < cur.own);
+    unsigned nb_mask = 3u;
+    if (cur.idx < cur.own) {
+      const int kl = cur.kc - c0;
+      contributes = contributes && (CT ? (kl + TW_KC > nj0) : (kl < nj0 + 32));
+      if constexpr (TW_KC == 16) {
+        const int rb = kl >> 4, cb0 = nj0 >> 4;
+        nb_mask = CT ? ((rb >= cb0 ? 1u : 0u) | (rb >= cb0 + 1 ? 2u : 0u))
+                     : ((rb <= cb0 ? 1u : 0u) | (rb <= cb0 + 1 ? 2u : 0u));
+      }
+    }
+    if (contributes && nb_mask == 3u) {
+      if constexpr (CPLX)
+        mma_steps_3m<T, 2, 2, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
+                                             TW_KC / 4, 1, p1, p2, p3);
+      else
+        mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
+                                            1, p1, p2);
+    } else if (contributes) {
+      if constexpr (CPLX)
+        mma_steps_3m<T, 2, 2, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
+                                             TW_KC / 4, 1, p1, p2, p3, nb_mask);
+      else
+        mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
+                                            1, p1, p2);
+    }
+    if (!last) {
+      next(cur);
+      return true;
+    }
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        acc_t re = p1[mb][nb], im = p2[mb][nb];
+        if constexpr (CPLX) combine3m<PROD_ADD>(p1[mb][nb], p2[mb][nb], p3[mb][nb], re, im);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t orow = row0 + mi0 + 16 * mb + Mfma<T>::row(lk, r);
+          const int col = c0 + nj0 + 16 * nb + li;
+          if (orow < m && col < k) store_elem<TS, CPLX>(Y, orow + (int64_t)col * ldy, (TS)re[r], (TS)im[r]);
+        }
+        p1[mb][nb] = Mfma<T>::zero();
+        p2[mb][nb] = Mfma<T>::zero();
+        p3[mb][nb] = Mfma<T>::zero();
+      }
+    next(cur);
+    return cur.valid;
+  };
+  while (slice(std::integral_constant<int, 0>()) && slice(std::integral_constant<int, 1>())) {
   }
 }
 
@@ -419,8 +642,12 @@ static int launch_wide(TS *Y, int64_t ldy, int64_t m, int k, const T *M, hipStre
   const int64_t kp = trsm_kp(k);
   const int64_t grid = cdiv64(m, TW_ROWS);
   if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
-  hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y, ldy,
-                     m, k, M, kp);
+  if constexpr (CT)
+    hipLaunchKernelGGL((trsm_wide_short_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream,
+                       Y, ldy, m, k, M, kp);
+  else
+    hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y,
+                       ldy, m, k, M, kp);
   return check_launch("trsm_wide_kernel");
 }
 
