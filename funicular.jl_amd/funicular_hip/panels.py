@@ -636,8 +636,9 @@ def _collect_columns(pm, cols, out):
         dst = out[:, lo - cols.start:hi - cols.start]
         if isinstance(panel.tier, GhostTier):
             tmp = backend.alloc_device(pm.dtype, (pm.N, hi - lo))
+            # The fill and the blocking copy back share the calling thread's
+            # stream, so the copy sees the fill without a wait of its own.
             backend.ghost_fill(tmp, 0, panel.home.cols[0] + (lo - pr.start), panel.home.source.seed)
-            backend.sync_all()
             host = np.empty((pm.N, hi - lo), dtype=pm.dtype, order="F")
             backend.d2h(host, tmp)
             dst[...] = host

@@ -197,3 +197,36 @@ def test_the_fused_schedule_is_an_opt_in(backend):
         ups[schedule] = moved["h2d_bytes"] // (N * k * 16)
     assert ups == {"reference": 4, "fused": 3}
     assert np.linalg.norm(Rs["fused"] - Rs["reference"]) <= 1e-13 * np.linalg.norm(Rs["reference"])
+
+
+@pytest.mark.parametrize("schedule", ("reference", "fused"))
+def test_no_host_wait_inside_a_sweeps_issue_loop(backend, schedule):
+    # The library counts every blocking wait (fhip_get_counters host_syncs).
+    # Read at each step a sweep issues, it must not move within a sweep: the
+    # issue loop queues work and waits only once, after its last step
+    # (test/test_discipline.jl:13-45; src/executor.jl:193-226).
+    N, k, w = 300000, 64, 8
+    A = _normal(np.random.default_rng(4107), N, k)
+    plan = _plan(backend, schedule=schedule)
+    Y = F.PanelMatrix.from_array(A, plan=plan, w=w)
+    seen = []
+    originals = {name: getattr(backend, name) for name in ("gram_accumulate", "rdiv_upper", "rdiv_upper_folded")}
+
+    def watched(name):
+        def call(*args, **kwargs):
+            seen.append((len(records), _lib.counters()["host_syncs"]))
+            return originals[name](*args, **kwargs)
+        return call
+    try:
+        for name in originals:
+            setattr(backend, name, watched(name))
+        with trace_sweeps() as records:
+            F.cholqr2(Y)
+    finally:
+        for name, fn in originals.items():
+            setattr(backend, name, fn)
+    by_sweep = {}
+    for sweep_index, syncs in seen:
+        by_sweep.setdefault(sweep_index, set()).add(syncs)
+    assert len(by_sweep) >= 3
+    assert all(len(values) == 1 for values in by_sweep.values()), by_sweep
