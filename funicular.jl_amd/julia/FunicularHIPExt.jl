@@ -63,7 +63,17 @@ Base.view(A::HIPMatrix, rows::AbstractUnitRange, ::Colon) = view(A, rows, 1:size
 
 devicetype(::HIPBackend, ::Type{T}) where {T} = HIPMatrix{T}
 supports_eltype(::HIPBackend, ::Type{T}) where {T} = T in (Float32, Float64, ComplexF32, ComplexF64)
-ka_backend(::HIPBackend) = nothing  # no KernelAbstractions kernels run on this backend
+
+# Funicular reaches ka_backend for two things: to launch its KernelAbstractions
+# kernels (src/kernels.jl), which this backend replaces with the library's
+# (convert_copy!, gram_accumulate!, rightmul_gemm!, rdiv_upper! below), and to
+# wait after a bare convert_staged! (src/panels.jl:365-371). KernelAbstractions
+# has no AMD backend without AMDGPU.jl, so HIPKernels is a handle whose
+# synchronize drains the device; nothing is ever launched through it.
+const KA = Funicular.KernelAbstractions
+struct HIPKernels <: KA.GPU end
+KA.synchronize(::HIPKernels) = (check(ccall((:fhip_device_sync, LIB[]), Cint, ())); nothing)
+ka_backend(::HIPBackend) = HIPKernels()
 
 function alloc_device(::HIPBackend, ::Type{T}, dims::Dims) where {T}
     d2 = length(dims) == 1 ? (dims[1], 1) : dims
@@ -134,50 +144,88 @@ function region(A::StridedMatrix{T}) where {T}
     (Ptr{Cvoid}(pointer(A)), stride(A, 2) * sizeof(T))
 end
 
-function copy2d!(dst, src, kind, queue)
+# A byte copy between two arrays of one eltype, queued on `stream`.
+function raw_copy!(dst, src, kind, stream)
     size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    eltype(dst) === eltype(src) ||
+        throw(ArgumentError("a byte copy needs one eltype on both sides, got $(eltype(src)) into $(eltype(dst)); " *
+                            "h2d! and d2h! convert through a device staging buffer"))
     isempty(dst) && return nothing
     T = eltype(dst)
     dp, dpitch = region(dst)
     sp, spitch = region(src)
-    stream = queue === nothing ? C_NULL : queue.stream
     GC.@preserve dst src check(ccall((:fhip_memcpy2d_async, LIB[]), Cint,
         (Ptr{Cvoid}, Csize_t, Ptr{Cvoid}, Csize_t, Csize_t, Csize_t, Cint, Ptr{Cvoid}),
         dp, dpitch, sp, spitch, size(dst, 1) * sizeof(T), size(dst, 2), kind, stream))
+    nothing
+end
+
+function copy2d!(dst, src, kind, queue)
+    raw_copy!(dst, src, kind, queue === nothing ? C_NULL : queue.stream)
     queue === nothing && check(ccall((:fhip_stream_sync, LIB[]), Cint, (Ptr{Cvoid},), C_NULL))
     nothing
 end
 
-h2d!(dst::HIPMatrix, src, b::HIPBackend; queue=nothing) =
-    (copy2d!(dst, src, FHIP_H2D, queue); queue === nothing ? nothing : record_event(b, queue))
-d2h!(dst, src::HIPMatrix, b::HIPBackend; queue=nothing) =
-    (copy2d!(dst, src, FHIP_D2H, queue); queue === nothing ? nothing : record_event(b, queue))
+# The contract lets h2d! and d2h! change the eltype (src/device_api.jl:31-44).
+# Outside a sweep there is no staging buffer of the source eltype, so, as the
+# CUDA extension does (ext/FunicularCUDAExt.jl:195-212), a converting copy
+# allocates one on the device, moves the bytes, converts with the library's
+# kernel on the same stream, and waits before the buffer is let go.
+function converting_copy!(dst, src, b::HIPBackend, queue)
+    size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    stream = queue === nothing ? C_NULL : queue.stream
+    staging = nothing
+    task_local_storage(:fhip_stream, stream) do
+        if dst isa HIPMatrix && src isa HIPMatrix
+            convert_copy!(b, dst, src)
+        elseif dst isa HIPMatrix
+            staging = alloc_device(b, eltype(src), size(src))
+            raw_copy!(staging, src, FHIP_H2D, stream)
+            convert_copy!(b, dst, staging)
+        else
+            staging = alloc_device(b, eltype(dst), size(dst))
+            convert_copy!(b, staging, src)
+            raw_copy!(dst, staging, FHIP_D2H, stream)
+        end
+    end
+    GC.@preserve staging check(ccall((:fhip_stream_sync, LIB[]), Cint, (Ptr{Cvoid},), stream))
+    queue === nothing ? nothing : record_event(b, queue)
+end
+
+function h2d!(dst::HIPMatrix, src, b::HIPBackend; queue=nothing)
+    eltype(dst) === eltype(src) || return converting_copy!(dst, src, b, queue)
+    copy2d!(dst, src, FHIP_H2D, queue)
+    queue === nothing ? nothing : record_event(b, queue)
+end
+
+function d2h!(dst, src::HIPMatrix, b::HIPBackend; queue=nothing)
+    eltype(dst) === eltype(src) || return converting_copy!(dst, src, b, queue)
+    copy2d!(dst, src, FHIP_D2H, queue)
+    queue === nothing ? nothing : record_event(b, queue)
+end
 
 # The kernel overrides (the Metal extension's pattern, ext/FunicularMetalExt.jl:187-189).
-function gram_accumulate!(::HIPBackend, C::HIPMatrix{T}, A::HIPMatrix{T}, B::HIPMatrix{T}, α, β) where {T}
+gram_accumulate!(::HIPBackend, C::HIPMatrix{T}, A::HIPMatrix{T}, B::HIPMatrix{T}, α, β) where {T} =
+    gram_raw!(C, A, B, α, β, eltcode(T))
+
+# C <- α A^H B + β C with the products in eltype `compute` (FHIP_C32 on
+# ComplexF64 storage is the mixed first pass: FP32 products, FP64 sums).
+function gram_raw!(C::HIPMatrix{T}, A::HIPMatrix{T}, B::HIPMatrix{T}, α, β, compute::Integer) where {T}
     herm = Cint(A.ptr == B.ptr && A.ld == B.ld && size(A, 2) == size(B, 2))
     n = Ref{Csize_t}(0)
     check(ccall((:fhip_gram_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Int64, Int64, Cint, Cint, Cint),
-                n, size(A, 1), size(A, 2), size(B, 2), eltcode(T), eltcode(T), herm))
+                n, size(A, 1), size(A, 2), size(B, 2), eltcode(T), compute, herm))
     work = workspace(n[])
     a, bb = ComplexF64(α), ComplexF64(β)
     check(ccall((:fhip_gram, LIB[]), Cint,
         (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Int64, Float64, Float64, Float64, Float64,
          Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
         C.ptr, C.ld, A.ptr, A.ld, B.ptr, B.ld, size(A, 1), size(A, 2), size(B, 2), real(a), imag(a), real(bb), imag(bb),
-        eltcode(T), eltcode(T), work.ptr, n[], current_stream()))
+        eltcode(T), compute, work.ptr, n[], current_stream()))
     C
 end
 
-function rdiv_upper!(::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}) where {T}
-    n = Ref{Csize_t}(0)
-    check(ccall((:fhip_trsm_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Cint), n, size(R, 1), eltcode(T)))
-    work = workspace(n[]; which=TRSM_WORK)
-    check(ccall((:fhip_trsm_runn, LIB[]), Cint,
-        (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
-        Y.ptr, Y.ld, size(Y, 1), size(Y, 2), R.ptr, R.ld, eltcode(T), eltcode(T), work.ptr, n[], current_stream()))
-    Y
-end
+rdiv_upper!(::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}) where {T} = trsm_raw!(Y, R, 1, eltcode(T))
 
 # With the block index passed to rdiv_upper! by rdiv_rows! (src/panelblas.jl:157-169;
 # the default method for other backends drops it), R is folded into the solve's
@@ -186,15 +234,54 @@ end
 # pass) does not overwrite the folded factor. Every block of one rdiv_rows!
 # runs on the one compute stream, so the fold is complete before a later block
 # reads it.
-function rdiv_upper!(b::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}, block::Integer) where {T}
-    block == 1 && return rdiv_upper!(b, Y, R)
+rdiv_upper!(::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}, block::Integer) where {T} =
+    trsm_raw!(Y, R, block, eltcode(T))
+
+# Y <- Y R^-1 with the products in eltype `compute`; block 1 folds R into the
+# solve's workspace, later blocks of the same sweep reuse it.
+function trsm_raw!(Y::HIPMatrix{T}, R::HIPMatrix{T}, block::Integer, compute::Integer) where {T}
+    n = Ref{Csize_t}(0)
+    check(ccall((:fhip_trsm_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Cint), n, size(R, 1), compute))
+    work = workspace(n[]; which=TRSM_WORK)
+    if block == 1
+        check(ccall((:fhip_trsm_runn, LIB[]), Cint,
+            (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+            Y.ptr, Y.ld, size(Y, 1), size(Y, 2), R.ptr, R.ld, eltcode(T), compute, work.ptr, n[], current_stream()))
+    else
+        check(ccall((:fhip_trsm_apply, LIB[]), Cint,
+            (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+            Y.ptr, Y.ld, size(Y, 1), size(Y, 2), eltcode(T), compute, Cint(0), work.ptr, n[], current_stream()))
+    end
+    Y
+end
+
+# Y <- Y R^-H (R upper triangular), the solve of the QR adjoint, folded once
+# per sweep in the same way. No counterpart in the reference.
+function rdiv_upper_adjoint!(::HIPBackend, Y::HIPMatrix{T}, R::HIPMatrix{T}, block::Integer=1) where {T}
     n = Ref{Csize_t}(0)
     check(ccall((:fhip_trsm_workspace_bytes, LIB[]), Cint, (Ptr{Csize_t}, Int64, Cint), n, size(R, 1), eltcode(T)))
     work = workspace(n[]; which=TRSM_WORK)
-    check(ccall((:fhip_trsm_apply, LIB[]), Cint,
-        (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
-        Y.ptr, Y.ld, size(Y, 1), size(Y, 2), eltcode(T), eltcode(T), Cint(0), work.ptr, n[], current_stream()))
+    if block == 1
+        check(ccall((:fhip_trsm_runc, LIB[]), Cint,
+            (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+            Y.ptr, Y.ld, size(Y, 1), size(Y, 2), R.ptr, R.ld, eltcode(T), work.ptr, n[], current_stream()))
+    else
+        check(ccall((:fhip_trsm_apply, LIB[]), Cint,
+            (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Cint, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+            Y.ptr, Y.ld, size(Y, 1), size(Y, 2), eltcode(T), eltcode(T), Cint(1), work.ptr, n[], current_stream()))
+    end
     Y
+end
+
+# out[i] (+)= Re sum_j conj(A[i, j]) B[i, j]: the sensitivity of S = Q^H G Q
+# to a real diagonal of G, one panel at a time. No counterpart in the reference.
+function rowdot!(::HIPBackend, out::HIPMatrix{Float64}, A::HIPMatrix{T}, B::HIPMatrix{T}, accumulate::Bool) where {T}
+    size(A) == size(B) && size(out, 1) >= size(A, 1) ||
+        throw(DimensionMismatch("rowdot of $(size(A)) and $(size(B)) into $(size(out))"))
+    check(ccall((:fhip_rowdot, LIB[]), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Ptr{Cvoid}),
+        out.ptr, A.ptr, A.ld, B.ptr, B.ld, size(A, 1), size(A, 2), eltcode(T), Cint(accumulate), current_stream()))
+    out
 end
 
 function rightmul_gemm!(::HIPBackend, dst::HIPMatrix{T}, A::HIPMatrix{T}, C::HIPMatrix{T}) where {T}
@@ -294,6 +381,76 @@ function factor_gram!(b::HIPBackend, R::HIPMatrix{T}, G::HIPMatrix{T}, N::Intege
     R_host, shifted = Funicular.cholesky_upper(G_host, N, size(G, 1))  # the reference's retry and shift
     copy2d!(R, R_host, FHIP_H2D, nothing)
     shifted
+end
+
+# cholqr2! with a mixed-precision first pass, on the reference's step order
+# (src/panelblas.jl:130-146): the first pass forms its Gram on the FP32 matrix
+# cores with FP64 sums across row chunks, factors it in FP32 on the host
+# (unshifted first with shift=:adaptive, or always with the shift
+# 11 (N k + k (k + 1)) u32 ||Y||_2^2 with shift=:fukaya) and divides by that
+# factor with FP32 products (solve=:fp32) or FP64 ones (solve=:fp64); the
+# second pass is the reference's FP64 cholqr_pass!, and a third runs when
+# either factorization was shifted. Returns R with Q R = Y; Y holds Q. No
+# counterpart in the reference.
+function cholqr2_mixed!(Y::Funicular.PanelMatrix{ComplexF64}; nbuffers=nothing, shift::Symbol=:adaptive,
+                        solve::Symbol=:fp32)
+    shift in (:adaptive, :fukaya) || throw(ArgumentError("shift must be :adaptive or :fukaya, got $shift"))
+    solve in (:fp32, :fp64) || throw(ArgumentError("solve must be :fp32 or :fp64, got $solve"))
+    Y.plan.backend isa HIPBackend || throw(ArgumentError("cholqr2_mixed! runs on the HIP backend"))
+    G = gram_fp32(Y; nbuffers=nbuffers)
+    R32, s = cholesky_upper_fp32(G, Y.N, Y.k, shift)
+    R1 = Matrix{ComplexF64}(R32)
+    rdiv_rows_compute!(Y, R1, solve === :fp32 ? FHIP_C32 : FHIP_C64; nbuffers=nbuffers)
+    R2, shifted2 = Funicular.cholqr_pass!(Y, nbuffers)
+    R = R2 * R1
+    if s > 0 || shifted2
+        R3, _ = Funicular.cholqr_pass!(Y, nbuffers)
+        R = R3 * R
+    end
+    R
+end
+
+# The first pass's Gram: gram's row sweep (src/panelblas.jl:33-53) with FP32
+# products.
+function gram_fp32(Y::Funicular.PanelMatrix{ComplexF64}; nbuffers=nothing)
+    plan = Y.plan
+    acc = only(Funicular.checkout_device_buffers!(plan, ComplexF64, (Y.k, Y.k), 1))
+    try
+        Funicular.sweep!(Funicular.readpanels(Y); nbuffers=nbuffers, traversal=Funicular.rowtraversal(Y)) do block, Yb
+            gram_raw!(acc, Yb, Yb, 1.0, block == 1 ? 0.0 : 1.0, FHIP_C32)
+        end
+        Funicular.collect_small(acc, plan)
+    finally
+        Funicular.checkin_device_buffers!(plan, (acc,))
+    end
+end
+
+function cholesky_upper_fp32(G::Matrix{ComplexF64}, N::Integer, k::Integer, shift::Symbol)
+    if shift === :adaptive
+        plain = cholesky(Hermitian(Matrix{ComplexF32}(G)); check=false)
+        issuccess(plain) && return Matrix(plain.U), 0.0
+    end
+    s = 11 * (N * k + k * (k + 1)) * (eps(Float32) / 2) * eigmax(Hermitian(G))
+    shifted = cholesky(Hermitian(Matrix{ComplexF32}(G + s * I)); check=false)
+    issuccess(shifted) || throw(ArgumentError("the $(k)×$(k) FP32 Gram matrix is not positive definite after " *
+                                              "the shift $s; the columns are numerically dependent"))
+    Matrix(shifted.U), s
+end
+
+# rdiv_rows! (src/panelblas.jl:157-169) with the solve's products in `compute`.
+function rdiv_rows_compute!(Y::Funicular.PanelMatrix{ComplexF64}, R::Matrix{ComplexF64}, compute::Integer;
+                            nbuffers=nothing)
+    plan = Y.plan
+    factor = only(Funicular.checkout_device_buffers!(plan, ComplexF64, size(R), 1))
+    try
+        h2d!(factor, R, plan.backend)
+        Funicular.sweep!(Funicular.updatepanels(Y); nbuffers=nbuffers, traversal=Funicular.rowtraversal(Y)) do block, Yb
+            trsm_raw!(Yb, factor, block, compute)
+        end
+    finally
+        Funicular.checkin_device_buffers!(plan, (factor,))
+    end
+    Y
 end
 
 # Workspaces that grow to the largest request and are reused: the first
