@@ -219,7 +219,7 @@ def slab_matrix(slab, T, dims, offset):
         raise ArgumentError(f"slab offset must be nonnegative, got {offset}")
     if offset + nbytes > slab.nbytes:
         raise ArgumentError(
-            f"a {human_readable_bytes(nbytes)} view at offset {offset} does not fit in a "
+            f"{human_readable_bytes(nbytes)} starting {offset} bytes in runs past the end of a "
             f"{human_readable_bytes(slab.nbytes)} host slab")
     if nbytes == 0:
         return np.zeros((rows, cols), dtype=T, order="F")
@@ -244,12 +244,12 @@ def host_region(A):
     if A.ndim == 1:
         A = A.reshape(-1, 1)
     if A.ndim != 2:
-        raise ArgumentError("Funicular copies panels and blocks of them, which are at most two-dimensional")
+        raise ArgumentError(f"copies move matrices (panels and row blocks); got an array of {A.ndim} dimensions")
     item = A.dtype.itemsize
     if A.shape[0] > 1 and A.strides[0] != item:
         raise ArgumentError(
-            f"a copy needs its columns contiguous but this array has a stride of {A.strides[0] // item} "
-            "between neighbouring rows")
+            f"copies need each column contiguous in memory, but consecutive rows of this array are "
+            f"{A.strides[0] // item} elements apart")
     ld = A.strides[1] // item if A.shape[1] > 1 else max(A.shape[0], 1)
     return A.ctypes.data, max(ld, A.shape[0])
 
@@ -410,7 +410,7 @@ class HIPBackend(DeviceBackend):
         if src.ndim == 1:
             src = src.reshape(-1, 1)
         if tuple(dst.shape) != tuple(src.shape):
-            raise DimensionMismatch(f"copy between a {dst.shape} destination and a {src.shape} source")
+            raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
         if dst.dtype != src.dtype:
             return self._converting_copy(dst, src, queue, to_device=True)
         return self._copy(dst.ptr, dst.ld, src, _lib.FHIP_H2D, queue, host_is_src=True, dev=dst)
@@ -420,14 +420,14 @@ class HIPBackend(DeviceBackend):
         if dst.ndim == 1:
             dst = dst.reshape(-1, 1)
         if tuple(dst.shape) != tuple(src.shape):
-            raise DimensionMismatch(f"copy between a {dst.shape} destination and a {src.shape} source")
+            raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
         if dst.dtype != src.dtype:
             return self._converting_copy(src, dst, queue, to_device=False)
         return self._copy(src.ptr, src.ld, dst, _lib.FHIP_D2H, queue, host_is_src=False, dev=src)
 
     def d2d(self, dst, src, queue=None):
         if tuple(dst.shape) != tuple(src.shape):
-            raise DimensionMismatch(f"copy between a {dst.shape} destination and a {src.shape} source")
+            raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
         if dst.dtype != src.dtype:
             self.convert_copy(dst, src, stream=_stream_of(queue) if queue is not None else current_stream())
             return self._finish(queue)
@@ -601,7 +601,7 @@ class HIPBackend(DeviceBackend):
     def convert_copy(self, dst, src, stream=None):
         """dst <- convert(eltype(dst), src) (src/kernels.jl:14-20)."""
         if tuple(dst.shape) != tuple(src.shape):
-            raise DimensionMismatch(f"copy between a {dst.shape} destination and a {src.shape} source")
+            raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
         if dst.size == 0:
             return dst
         s = current_stream() if stream is None else stream

@@ -3,12 +3,12 @@
 A sweep visits panel matrices step by step. Each step's read operands are
 copied into device staging buffers on the upload queue, the panel function runs
 on the compute queue, and write operands are copied back on the writeback
-queue, the three ordered by events only. With nbuffers=2 the upload of step
-s+1 overlaps the compute of step s; a buffer is reused only after the step that
-held it nbuffers steps earlier has finished with it. A producer thread runs
-FEED_DEPTH steps ahead, bringing each step's panels to where the copies can
-read them. The issue loop never synchronizes and never allocates: buffers come
-from the plan's pool and the host waits once, at the end.
+queue; only events order the three. With nbuffers=2 the uploads run one
+step ahead of the compute, and a slot's buffer is refilled only once the step that used
+it nbuffers steps before is done with it. A producer thread stays FEED_DEPTH
+steps ahead, bringing each step's panels to where the copies can read them.
+The issue loop allocates nothing (every buffer is the plan's) and never waits;
+the host blocks once, after the last step.
 
 Device-resident operands (plans with device_resident=True) need no staging:
 their steps are views of the matrix's own HBM allocation.
@@ -275,30 +275,31 @@ def sweep(f, *operands, nbuffers=None, traversal=None):
     """Calls f(label, *buffers) once per step with one device view per operand
     (src/executor.jl:165-239)."""
     if not operands:
-        raise ArgumentError("a sweep needs at least one panel matrix, wrapped as readpanels(pm), writepanels(pm) "
-                            "or updatepanels(pm)")
+        raise ArgumentError("sweep was given no operand; pass each panel matrix as readpanels(pm), "
+                            "writepanels(pm) or updatepanels(pm)")
     reference = operands[0].pm
     plan = reference.plan
     steps = PanelSteps(npanels(reference)) if traversal is None else traversal
     for op in operands:
         assert_sweepable(steps, reference, op.pm)
         if op.pm.plan is not plan:
-            raise ArgumentError("every panel matrix in one sweep must belong to the same ResidencyPlan, since the "
-                                "plan is what owns the staging buffers the sweep runs through")
+            raise ArgumentError("the operands of a sweep come from different ResidencyPlans; a sweep stages "
+                                "through one plan's buffers, so build its matrices from a single plan")
         if op.write:
             assert_writable(op.pm)
     for i in range(1, len(operands)):
         for l in range(i):
             if operands[i].pm is operands[l].pm:
-                raise ArgumentError("the same panel matrix takes part in one sweep twice. Pass it once as "
-                                    "updatepanels(pm) if the sweep both reads and writes it")
+                raise ArgumentError("a panel matrix appears twice among the operands of one sweep; to read and "
+                                    "write it in the same sweep, pass it once, as updatepanels(pm)")
     requested = plan.nbuffers if nbuffers is None else int(nbuffers)
     if requested < 1:
-        raise ArgumentError(f"nbuffers must be at least 1, got {requested}")
+        raise ArgumentError(f"a sweep needs one staging buffer or more per operand; nbuffers = {requested}")
     writing = any(op.write for op in operands)
     if writing and repeats_a_panel(steps):
-        raise ArgumentError("this sweep visits a panel more than once and writes back, so the second visit would "
-                            "overwrite the first. Repeated panels are only allowed in a read-only sweep")
+        raise ArgumentError("the traversal returns to a panel it has visited and the sweep writes back: the later "
+                            "write-back would replace the earlier one. A traversal may repeat panels only when "
+                            "every operand is read-only")
 
     backend = plan.backend
     count = nsteps(steps)

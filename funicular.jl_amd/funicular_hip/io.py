@@ -35,8 +35,8 @@ FORMAT = 1
 HEADER_BYTES = 4096
 DATASET = "panels"
 
-# Every call that touches a store's file goes through this lock; a producer
-# holds it for one panel-sized read and never while it waits on the pipeline
+# File access is serialized by this lock. It is held around a single panel's
+# read or write and released before the producer blocks on a free slot
 # (src/io.jl:63-66).
 DISK_LOCK = threading.RLock()
 SCRATCH_COUNTER = itertools.count()
@@ -50,8 +50,8 @@ _ELTYPE_NAMES = {v: k for k, v in STORAGE_ELTYPES.items()}
 def eltype_name(T):
     T = np.dtype(T)
     if T not in _ELTYPE_NAMES:
-        raise ArgumentError(f"the disk tier cannot store {T}. It holds {', '.join(sorted(STORAGE_ELTYPES))}; narrow "
-                            "the cold tiers to one of those or leave them at the compute eltype")
+        raise ArgumentError(f"{T} has no place on disk; panel files hold {', '.join(sorted(STORAGE_ELTYPES))}. "
+                            "Set the plan's storage eltypes to one of them, or keep the compute eltype")
     return _ELTYPE_NAMES[T]
 
 
@@ -59,7 +59,7 @@ def storage_eltype_named(name):
     """src/io.jl:75-78."""
     if name in STORAGE_ELTYPES:
         return STORAGE_ELTYPES[name]
-    raise ArgumentError(f"this file stores {name}, which is not an eltype Funicular reads. Known eltypes are "
+    raise ArgumentError(f"the panel file names its eltype {name!r}; readable ones: "
                         f"{', '.join(sorted(STORAGE_ELTYPES))}")
 
 
@@ -79,12 +79,12 @@ class RawStore:
 
 
 def open_panel_store(path, mode, *, N=0, k=0, w=0, stored=np.complex128, computed=np.complex128):
-    """Opens or creates the file behind a store (src/io.jl:5-18,
-    ext/FunicularHDF5Ext.jl:21-42). Mode "w" creates a file laid out for an
-    N x k matrix of eltype `stored`, one panel of w columns per chunk; "r" and
-    "r+" open an existing one, whose header fixes the layout."""
+    """The RawStore over the panel file at path (src/io.jl:5-18,
+    ext/FunicularHDF5Ext.jl:21-42). "w" truncates or makes the file and
+    sizes it for N x k entries of `stored`, chunked w columns at a time; "r"
+    and "r+" take N, k, w and the eltypes from the file's header."""
     if mode not in ("w", "r", "r+"):
-        raise ArgumentError(f'a panel store opens in mode "w", "r" or "r+", got {mode!r}')
+        raise ArgumentError(f'mode {mode!r}: panel files open as "w", "r" or "r+"')
     if mode == "w":
         N, k, w = int(N), int(k), int(w)
         if N <= 0 or k <= 0 or w <= 0:
@@ -108,7 +108,7 @@ def open_panel_store(path, mode, *, N=0, k=0, w=0, stored=np.complex128, compute
     try:
         raw = os.pread(fd, HEADER_BYTES, 0)
         if not raw.startswith(MAGIC):
-            raise ArgumentError(f"the file at {path!r} has no {DATASET!r} dataset, so it is not one Funicular wrote")
+            raise ArgumentError(f"{path!r} does not start with the panel-file magic: not one Funicular wrote")
         header = json.loads(raw[len(MAGIC):].rstrip(b"\0").decode())
         if header.get("funicular_format") != FORMAT:
             raise ArgumentError(f"the file at {path!r} is panel format {header.get('funicular_format')}, and this "
@@ -282,8 +282,7 @@ def read_panel(home, dst):
     """src/io.jl:111-120."""
     store = home.store
     if store.handle is None:
-        raise ArgumentError(f"the store at {store.path!r} is closed, so the panel that lived there cannot be read "
-                            "back. It was closed by free or by save")
+        raise ArgumentError(f"reading a panel from {store.path!r}, which free or save has closed already")
     with DISK_LOCK:
         read_panel_region(store.handle, dst, home.cols)
         store.reads += 1
@@ -294,11 +293,10 @@ def write_panel(home, src):
     """src/io.jl:122-132."""
     store = home.store
     if store.handle is None:
-        raise ArgumentError(f"the store at {store.path!r} is closed, so the panel cannot be written back to it")
+        raise ArgumentError(f"writing a panel back to {store.path!r}, which free or save has closed already")
     if store.readonly:
-        raise ArgumentError(f"the store at {store.path!r} was loaded read only and this panel has been modified, so "
-                            "there is nowhere to put it. Load the matrix without readonly=True if the operation is "
-                            "meant to reach the file")
+        raise ArgumentError(f"a modified panel cannot go back to {store.path!r}: the matrix was loaded read only. "
+                            "Load it with readonly=False to let operations change the file")
     with DISK_LOCK:
         write_panel_region(store.handle, src, home.cols)
         store.writes += 1
@@ -350,8 +348,8 @@ def cool(home, panel):
 
 
 def save(pm, path):
-    """Writes pm to `path` in the panel file format, in the storage eltype the
-    plan's cold tiers use, and leaves pm where it was (src/io.jl:146-172)."""
+    """A panel file at `path` holding pm's entries, converted to the plan's
+    disk storage eltype; pm itself does not move (src/io.jl:146-172)."""
     from .panels import _collect_columns, disk_storage_eltype, npanels, panelrange
     S = disk_storage_eltype(pm.plan, pm.dtype)
     store = open_store(path, "w", N=pm.N, k=pm.k, w=pm.w, stored=S, computed=pm.dtype)
@@ -367,26 +365,24 @@ def save(pm, path):
 
 
 def load(path, *, plan, readonly=False, dtype=None):
-    """A PanelMatrix whose panels start on disk, in the file at `path`
-    (src/io.jl:174-212). The file is the matrix's cold tier, not a copy: a
-    panel an operation modifies is written back into it (unless readonly,
-    which turns such a write into an error)."""
+    """A PanelMatrix backed by the panel file at `path` (src/io.jl:174-212).
+    Its panels live in that file: operations that change a panel change the
+    file, and with readonly=True such a change raises instead."""
     from .panels import Panel, PanelMatrix, host_storage_eltype
     if not os.path.isfile(path):
-        raise ArgumentError(f"there is no file at {path!r} to load a PanelMatrix from")
+        raise ArgumentError(f"load: no file at {path!r}")
     store = open_store(path, "r" if readonly else "r+")
     try:
         T = store.computed if dtype is None else np.dtype(dtype)
         S = host_storage_eltype(plan, T)
         if S != store.stored:
-            raise ArgumentError(f"the file at {path!r} stores {eltype_name(store.stored)} but this plan's cold tiers "
-                                f"hold {S}. Give the plan host_eltype = {store.stored}, or load the file with a "
-                                "plan whose storage eltype matches it")
+            raise ArgumentError(f"{path!r} stores {eltype_name(store.stored)} entries and the plan keeps {S} off the "
+                                f"device; load it with host_eltype = {store.stored}")
         if plan.device_resident:
             raise ArgumentError("a device-resident plan keeps its matrices in HBM; load the file with a plan "
                                 "that streams (device_resident=False)")
-        # The file's chunking fixes the width; this only refuses a plan whose
-        # device budget cannot stage a panel that wide.
+        # The width is the file's; the call below only checks that the plan
+        # can stage panels of it.
         _plan.resolve_panel_width(plan, store.N, store.k, T, override=store.w)
         panels = []
         for j in range(_plan.cld(store.k, store.w)):

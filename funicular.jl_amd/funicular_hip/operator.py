@@ -46,15 +46,15 @@ def operator_size(G):
 def assert_square_operator(G, N):
     rows, cols = operator_size(G)
     if not (rows == cols == N):
-        raise ArgumentError(f"operator is {rows}×{cols} but the panel matrices have {N} rows. Funicular applies an "
-                            "operator to whole columns, so it must be square and match N")
+        raise ArgumentError(f"this needs a square operator of order {N}, the row count of the panel matrices; "
+                            f"the operator is {rows}×{cols}")
 
 
 def assert_operator_shape(G, m, n):
     rows, cols = operator_size(G)
     if (rows, cols) != (int(m), int(n)):
-        raise ArgumentError(f"operator is {rows}×{cols} but it reads columns of an {n}-row matrix and writes "
-                            f"columns of an {m}-row one, so it has to be {m}×{n}")
+        raise ArgumentError(f"mapping columns of {n} rows onto columns of {m} rows takes an {m}×{n} operator; "
+                            f"this one is {rows}×{cols}")
 
 
 def _stencil_eltypes(G, Y, X):
@@ -239,7 +239,7 @@ def check_operator(G, n=3, backend=None, rtol=None, seed=None):
             backend = hip_backend()
     probes = int(n)
     if probes < 1:
-        raise ArgumentError(f"n must be at least 1, got {n}")
+        raise ArgumentError(f"check_operator needs one probe or more; n = {n}")
     rows, cols = operator_size(G)
     try:
         T = np.dtype(G.dtype)
@@ -248,12 +248,12 @@ def check_operator(G, n=3, backend=None, rtol=None, seed=None):
     if not backend.supports_eltype(T):
         raise ArgumentError(f"backend {type(backend).__name__} cannot compute in the operator's eltype {T}")
     if workspace_bytes(G) < 0:
-        raise ArgumentError(f"workspace_bytes(G) returned {workspace_bytes(G)}, which must be nonnegative")
+        raise ArgumentError(f"the operator reports a negative workspace ({workspace_bytes(G)} bytes)")
     adj = G.adjoint()
     adjdims = operator_size(adj)
     if adjdims != (cols, rows):
-        raise ArgumentError(f"adjoint(G) is {adjdims[0]}×{adjdims[1]} but G is {rows}×{cols}, so the two cannot "
-                            "be adjoints")
+        raise ArgumentError(f"G.adjoint() has shape {adjdims[0]}×{adjdims[1]}, not the transposed shape "
+                            f"{cols}×{rows} of G")
     real = np.finfo(T).eps
     tol = 64 * real * np.sqrt(max(rows, cols)) if rtol is None else rtol
     rng = np.random.default_rng(seed)
@@ -273,22 +273,23 @@ def check_operator(G, n=3, backend=None, rtol=None, seed=None):
         scale = max(abs(left), abs(right), 1.0)
         if not abs(left - right) <= max(tol * max(abs(left), abs(right)), tol * scale):
             raise ArgumentError(
-                f"G and adjoint(G) disagree on probe {c}: <y, Gx> = {left} but <G'y, x> = {right}, a relative "
-                f"difference of {abs(left - right) / max(abs(left), abs(right))} against a tolerance of {tol}")
+                f"probe {c}: <y, G x> = {left} while <G^H y, x> = {right} (relative gap "
+                f"{abs(left - right) / max(abs(left), abs(right))}, allowed {tol}), so G.adjoint() is not the "
+                "adjoint of G")
     if panel_capable(G):
         panelled = backend.alloc_device(T, (rows, probes))
         backend.fill(panelled, 0)
         try:
             G.apply(panelled, X)
         except Exception as err:
-            raise ArgumentError(f"G claims panel_capable but apply on a whole {cols}×{probes} panel threw: {err}")
+            raise ArgumentError(f"G sets panel_capable, yet applying it to a {cols}×{probes} block raised: {err}")
         P = backend.to_host(panelled)
         if not np.allclose(P, GXh, rtol=tol, atol=tol):
-            raise ArgumentError(f"G claims panel_capable but applying it to a whole {cols}×{probes} panel does not "
-                                f"match applying it column by column, to a relative tolerance of {tol}")
+            raise ArgumentError(f"G sets panel_capable, yet its result on a {cols}×{probes} block differs from "
+                                f"its column-by-column result by more than {tol} (relative)")
     if ishermitian_op(G):
         if rows != cols:
-            raise ArgumentError(f"G claims ishermitian_op but is {rows}×{cols}, which cannot be Hermitian")
+            raise ArgumentError(f"G sets ishermitian_op, yet it is {rows}×{cols}, not square")
         GY = backend.alloc_device(T, (rows, probes))
         backend.fill(GY, 0)
         columnwise_apply(GY, G, Y)
@@ -298,6 +299,6 @@ def check_operator(G, n=3, backend=None, rtol=None, seed=None):
             right = np.vdot(GYh[:, c], Xh[:, c])
             scale = max(abs(left), abs(right), 1.0)
             if not abs(left - right) <= tol * scale:
-                raise ArgumentError(f"G claims ishermitian_op but <y, Gx> = {left} and <Gy, x> = {right} differ on "
-                                    f"probe {c}, beyond a tolerance of {tol}")
+                raise ArgumentError(f"G sets ishermitian_op, yet probe {c} gives <y, G x> = {left} and "
+                                    f"<G y, x> = {right}, apart by more than {tol}")
     return True

@@ -34,11 +34,11 @@ def resolve_schedule(pm, schedule=None):
 def shared_plan(X, Y):
     """src/panelblas.jl:4-8."""
     if X.dtype != Y.dtype:
-        raise ArgumentError(f"panel matrices in one operation must share their compute eltype, got {X.dtype} and "
-                            f"{Y.dtype}. Narrowing belongs on the storage tiers, not here")
+        raise ArgumentError(f"the two panel matrices compute in {X.dtype} and {Y.dtype}; an operation takes one "
+                            "compute eltype, and a narrower eltype is set on the plan's storage tiers instead")
     if X.plan is not Y.plan:
-        raise ArgumentError("panel matrices in one operation must belong to the same ResidencyPlan, since the plan "
-                            "is what owns the staging buffers the operation runs through")
+        raise ArgumentError("the two panel matrices come from different ResidencyPlans; an operation stages "
+                            "through one plan's buffers, so build both from a single plan")
     return X.plan
 
 
@@ -55,8 +55,8 @@ def collect_small(device, plan):
 
 
 def gram(X, Y=None, *, nbuffers=None, compute=None):
-    """The k x k matrix X^H Y, accumulated on the device one row block at a
-    time and copied back once (src/panelblas.jl:10-53). `compute=np.complex64`
+    """The k x k matrix X^H Y: each row block adds its product to a device
+    accumulator, which comes to the host at the end (src/panelblas.jl:10-53). `compute=np.complex64`
     forms it on the FP32 matrix cores with FP64 sums across row chunks."""
     Y = X if Y is None else Y
     plan = shared_plan(X, Y)
@@ -97,8 +97,8 @@ def panelmul(Y, G, X, *, nbuffers=None):
     """Y_j <- G X_j for every column panel j (src/panelblas.jl:67-100)."""
     shared_plan(X, Y)
     if X is Y:
-        raise ArgumentError("panelmul cannot write its output over its input: the operator is applied panel by "
-                            "panel and would read columns it has already overwritten")
+        raise ArgumentError("panelmul needs distinct input and output matrices: applied panel by panel in "
+                            "place, the operator would meet columns that hold its own earlier output")
     assert_same_panels(X, Y)
     assert_operator_shape(G, Y.N, X.N)
     sweep(lambda _, Xb, Yb: apply_operator(Yb, G, Xb), readpanels(X), writepanels(Y), nbuffers=nbuffers)
@@ -107,7 +107,7 @@ def panelmul(Y, G, X, *, nbuffers=None):
 
 def cholqr2(Y, *, nbuffers=None, schedule=None):
     """Orthonormalizes Y in place by CholeskyQR2 and returns the k x k upper
-    triangular R with Q R equal to the original Y (src/panelblas.jl:102-139).
+    triangular R for which Q R reproduces the input (src/panelblas.jl:102-139).
     A pass whose Gram needed a shift triggers a third pass (shifted
     CholeskyQR3, Fukaya et al.).
 
@@ -318,9 +318,9 @@ def cholesky_upper(G, N, k):
     shifted = _potrf(G + shift * np.eye(G.shape[0], dtype=G.dtype))
     if shifted is None:
         raise ArgumentError(
-            f"the {k}×{k} Gram matrix is not positive definite even after a shift of {shift}, so the columns are "
-            f"numerically dependent: a zero column, a duplicated one, or a condition number past {1 / eps}. Drop "
-            "the dependent columns before orthonormalizing")
+            f"the Cholesky factorization of the {k}×{k} Gram failed again with the diagonal raised by {shift}: "
+            f"the columns are dependent to working precision (a zero or repeated column, or a condition number "
+            f"beyond {1 / eps}). Remove the dependent columns and orthonormalize the rest")
     return shifted, True
 
 
@@ -387,9 +387,9 @@ def rightmul(*args, nbuffers=None):
 def _rightmul_inplace(Y, C, nbuffers):
     C = np.asarray(C)
     if C.shape != (Y.k, Y.k):
-        raise ArgumentError(f"rightmul(Y, C) writes the product back into Y, which has {Y.k} columns and cannot grow "
-                            f"or shrink, so C must be {Y.k}×{Y.k}, got {C.shape}. Use rightmul(dest, Y, C) when the "
-                            "column count changes")
+        raise ArgumentError(f"rightmul(Y, C) overwrites Y, whose {Y.k} columns are fixed, so C has to be "
+                            f"{Y.k}×{Y.k}; it is {C.shape}. To change the number of columns, write the product into "
+                            "another matrix with rightmul(dest, Y, C)")
     from .panels import assert_writable
     assert_writable(Y)
     plan = Y.plan
@@ -415,12 +415,12 @@ def _rightmul_inplace(Y, C, nbuffers):
 
 def _rightmul_into(dest, src, C, nbuffers):
     if dest is src:
-        raise ArgumentError("rightmul(dest, src, C) needs two different matrices, since one of them streams up while "
-                            "the other streams down. Use rightmul(Y, C) when the product goes back where it came from")
+        raise ArgumentError("rightmul(dest, src, C) reads src while it writes dest, so they must be two matrices; "
+                            "for a product that replaces its own input call rightmul(Y, C)")
     C = np.asarray(C)
     if C.shape != (src.k, dest.k):
-        raise ArgumentError(f"rightmul(dest, src, C) computes src C, so C must be {src.k}×{dest.k} to take a "
-                            f"{src.N}×{src.k} source into a {dest.N}×{dest.k} destination, got {C.shape}")
+        raise ArgumentError(f"rightmul(dest, src, C) with a {src.N}×{src.k} src and a {dest.N}×{dest.k} dest "
+                            f"needs C to be {src.k}×{dest.k}; it is {C.shape}")
     plan = shared_plan(src, dest)
     assert_same_rows(src, dest)
     T = dest.dtype
@@ -716,8 +716,8 @@ def cholqr2_panelwise(Y, *, nbuffers=None, mixed=False, shift="adaptive", group=
     Two passes and a third when a Cholesky needed a shift, as cholqr2.
     mixed=True makes the first pass the mixed one of cholqr2_mixed with
     solve="fp64": its Gram on the FP32 matrix cores and its Cholesky in FP32
-    (shifted as `shift` says), the solve in FP64. Returns R with Q R equal to
-    the original Y; Y holds Q. `group` is the run length of the panel-wise
+    (shifted as `shift` says), the solve in FP64. Returns R such that Q R is
+    the input; Y holds Q afterwards. `group` is the run length of the panel-wise
     sweeps (panel_group's by default)."""
     def one_pass():
         G = gram_panelwise(Y, nbuffers=nbuffers, group=group)
@@ -755,7 +755,7 @@ def axpy(alpha, X, Y, *, nbuffers=None):
     shared_plan(X, Y)
     assert_conformal(X, Y)
     if X is Y:
-        raise ArgumentError("axpy(alpha, X, X) doubles a matrix against itself. Use scale(X, 1 + alpha)")
+        raise ArgumentError("axpy with X and Y the same matrix is a scaling; call scale(X, 1 + alpha)")
     if not np.issubdtype(Y.dtype, np.complexfloating) and complex(alpha).imag != 0:
         raise ArgumentError(f"cannot add a complex multiple ({alpha}) of a real {Y.dtype} panel matrix")
     backend = Y.plan.backend
@@ -802,7 +802,7 @@ def norm(X, p=2, *, nbuffers=None):
     the rescaling LinearAlgebra does for arrays (src/panelblas.jl:318-322,
     356-370)."""
     if p != 2:
-        raise ArgumentError(f"a PanelMatrix has only its Frobenius norm, p = 2, got p = {p}")
+        raise ArgumentError(f"norm of a PanelMatrix is the Frobenius norm (p = 2) only; p = {p} was asked for")
     R = np.finfo(X.dtype).dtype.type
     return R(np.sqrt(_reduce(X, None, nbuffers).real))
 

@@ -69,8 +69,8 @@ def host_storage_eltype(plan, T):
     S = plan.host_eltype
     if np.issubdtype(T, np.complexfloating) and not np.issubdtype(S, np.complexfloating):
         raise ArgumentError(
-            f"host_eltype {S} cannot hold the imaginary part of a {T} panel. Narrowing a complex compute eltype "
-            "means a narrower complex one")
+            f"a real host_eltype ({S}) would drop the imaginary parts of {T} panels; narrow a complex compute "
+            "eltype to a complex storage eltype such as complex64")
     return S
 
 
@@ -81,8 +81,8 @@ def disk_storage_eltype(plan, T):
         return S
     if plan.disk_eltype != S:
         raise ArgumentError(
-            f"disk_eltype {plan.disk_eltype} and the host tier's {S} disagree, and a panel moves between the two "
-            f"without conversion. Set host_eltype to {plan.disk_eltype} as well, or leave disk_eltype unset")
+            f"panels move between disk and host memory byte for byte, so disk_eltype ({plan.disk_eltype}) must "
+            f"equal the host eltype ({S}); give host_eltype the same value, or drop disk_eltype")
     return S
 
 
@@ -130,13 +130,13 @@ class PanelMatrix:
             return
         S = host_storage_eltype(plan, T)
         D = disk_storage_eltype(plan, T)
-        # A plan with a scratch_dir gives the matrix a file of its own, and
-        # panels spill to it when the host tier runs out (src/panels.jl:57-61).
+        # With a scratch_dir each matrix is backed by its own file, the place
+        # its panels go when host memory runs short (src/panels.jl:57-61).
         if plan.scratch_dir is not None:
             from .io import DiskHome, scratch_store
             self.store = scratch_store(plan, self.N, self.k, width, D, T)
-        # Every panel takes a block sized for the nominal width, ragged one
-        # included, so a released block fits the next panel (src/panels.jl:64-67).
+        # Blocks are all sized for a full-width panel, the narrower last one
+        # too, so any released block can take any panel (src/panels.jl:64-67).
         blockbytes = _plan.panel_bytes(self.N, width, S)
         try:
             for j in range(npan):
@@ -225,34 +225,34 @@ def assert_writable(pm):
     """src/ghosts.jl:81-84."""
     if isghost(pm):
         raise ArgumentError(
-            "this is a ghost matrix: its panels are regenerated from a seed rather than stored, so anything written "
-            "to one would be lost the moment that panel left memory. Write into a panel matrix of its own instead")
+            "a ghost matrix cannot be written: its panels are recomputed from the seed whenever they are needed, "
+            "so written values would not survive. Copy it into an ordinary PanelMatrix and write to that")
 
 
 def assert_conformal(X, Y):
     if (X.N, X.k, X.w) != (Y.N, Y.k, Y.w):
         raise ArgumentError(
-            f"panel matrices must share their (N, k, w) partitioning, got ({X.N}, {X.k}, {X.w}) and "
+            f"the two panel matrices are partitioned differently: (N, k, w) = ({X.N}, {X.k}, {X.w}) against "
             f"({Y.N}, {Y.k}, {Y.w})")
 
 
 def assert_same_panels(X, Y):
     if (X.k, X.w) != (Y.k, Y.w):
         raise ArgumentError(
-            f"this operation goes panel by panel, so the matrices must be cut into the same panels, but they have "
-            f"(k, w) = ({X.k}, {X.w}) and ({Y.k}, {Y.w})")
+            f"a panel-by-panel operation pairs panel j of one matrix with panel j of the other, which needs the same "
+            f"(k, w); here they are ({X.k}, {X.w}) and ({Y.k}, {Y.w})")
 
 
 def assert_same_rows(X, Y):
     if X.N != Y.N:
         raise ArgumentError(
-            f"this operation goes row block by row block, so the matrices must have the same rows, but they have "
-            f"{X.N} and {Y.N}")
+            f"a row-block operation pairs equal row ranges of the two matrices, which needs equal N; here it is "
+            f"{X.N} against {Y.N}")
 
 
 def checkout_staging_buffers(plan, pm, dims, count):
     """src/panels.jl:114-122: buffers in the storage eltype for a narrowed host tier."""
-    if pm.resident or isghost(pm):
+    if pm.resident or (isghost(pm) and pm.panels[0].home.source.generator is None):
         return None
     S = host_storage_eltype(plan, pm.dtype)
     if S == pm.dtype:
@@ -274,14 +274,13 @@ def convert_staged(dst, src, plan, queue, event):
 
 def _cold_panel_error():
     return ArgumentError(
-        "this copy reached a panel that is not in host memory: it is still on disk. Bring it up with prefetch "
-        "first, or let a sweep do it: a sweep stages every panel of a step before it issues a copy for it")
+        "the panel to copy is on disk, not in host memory. prefetch it first, or copy it inside a sweep, whose "
+        "producer stages each step's panels before the step's copies are issued")
 
 
 def _not_resident_error():
     return ArgumentError(
-        "this panel has no memory behind it: it was released by free, or it went to a colder tier and nothing "
-        "staged it back")
+        "the panel holds no data: free released it, or it was moved to a colder tier and not staged since")
 
 
 def _ghost_fill(dst, panel, row0, queue, backend):
@@ -308,8 +307,8 @@ def materialize(dst, panel, plan, queue=None, via=None):
         raise _not_resident_error()
     src = panelstorage(panel)
     if tuple(dst.shape) != tuple(src.shape):
-        raise DimensionMismatch(f"device buffer is {dst.shape} but the panel is {src.shape}; a ragged last panel "
-                                "needs a view of the staging buffer")
+        raise DimensionMismatch(f"the device buffer is {dst.shape} and the panel {src.shape}; copy a narrower "
+                                "last panel into a column view of the buffer")
     if isinstance(panel.tier, DeviceTier):
         return backend.d2d(dst, src, queue=queue)
     if via is None:
@@ -329,8 +328,8 @@ def writeback(panel, src, plan, queue=None, via=None):
         raise _not_resident_error()
     dst = panelstorage(panel)
     if tuple(dst.shape) != tuple(src.shape):
-        raise DimensionMismatch(f"device buffer is {src.shape} but the panel is {dst.shape}. A ragged last panel "
-                                "needs a view of the staging buffer")
+        raise DimensionMismatch(f"the device buffer is {src.shape} and the panel {dst.shape}; copy a narrower "
+                                "last panel out of a column view of the buffer")
     if isinstance(panel.tier, DeviceTier):
         event = backend.d2d(dst, src, queue=queue)
     else:
@@ -420,15 +419,38 @@ def writeback_rows(pm, src, rows, queue=None, via=None):
 
 def stage_host(panel, plan):
     """Bring a panel to where a sweep can copy from it and pin it there
-    (src/panels.jl:475-505, src/ghosts.jl:105-122). Ghost panels are generated
-    on the device by the copy itself, so staging one only pins it."""
+    (src/panels.jl:475-505, src/ghosts.jl:105-122). A ghost panel on the
+    device stream is generated by the copy itself, so staging it only pins
+    it; one with a host generator is generated into a host block here, as
+    the reference does, and from then on is a host panel until spilled."""
     if isinstance(panel.tier, (HostTier, DeviceTier)):
         if not isresident(panel):
-            raise ArgumentError("panel has no storage: it was released")
+            raise ArgumentError("this panel's storage was released (free, or a spill with nothing to bring it "
+                                "back)")
         _plan.pin(plan, panel)
         return panel
     if isinstance(panel.tier, GhostTier):
         _plan.pin(plan, panel)
+        source = panel.home.source
+        if source.generator is None:
+            return panel
+        from .ghosts import generate_panel
+        S = host_storage_eltype(plan, source.dtype)
+        try:
+            hs = _plan.alloc_host_storage(plan, S, (source.N, panel.width),
+                                          _plan.panel_bytes(source.N, source.w, S))
+        except BaseException:
+            _plan.unpin(plan, panel)
+            raise
+        try:
+            generate_panel(panel.home, hs.array)
+        except BaseException:
+            _plan.release_host_block(plan, hs.block)
+            _plan.unpin(plan, panel)
+            raise
+        panel.data = hs
+        panel.tier = HOST
+        _plan.register_resident(plan, panel)
         return panel
     if isinstance(panel.tier, DiskTier):
         from .io import stage_host_disk
@@ -468,11 +490,11 @@ def evict(panel, plan):
         raise ArgumentError("a device-resident panel has no colder tier to go to")
     if panel.home is None:
         raise ArgumentError(
-            f"panel lives on the host tier and the plan has no colder tier (scratch_dir = {plan.scratch_dir!r}), so "
-            "there is nowhere to evict it to")
+            f"evicting a host panel needs a colder tier, and this plan has none (scratch_dir = "
+            f"{plan.scratch_dir!r})")
     if panel.pins != 0:
-        raise ArgumentError("panel is staged for a sweep that is still running, so its host memory is not free to "
-                            "give back. Evict it once the sweep has returned")
+        raise ArgumentError("the panel is pinned by a sweep in progress, which may still copy from its host "
+                            "block; evict it after that sweep returns")
     spill(plan, panel)
 
 
@@ -503,7 +525,7 @@ def free(pm):
 # ------------------------------------------------------------ host copies --
 
 def copycols(dest, dcols, src, scols=None):
-    """Copies whole columns into columns dcols of dest (src/panels.jl:211-304),
+    """Whole columns into columns dcols of dest (src/panels.jl:211-304),
     from a dense host array or from columns scols of another PanelMatrix.
     Ranges are Python ranges (0-based)."""
     assert_writable(dest)
@@ -511,13 +533,14 @@ def copycols(dest, dcols, src, scols=None):
     _assert_column_range(dcols, dest.k, "destination")
     if isinstance(src, PanelMatrix):
         if dest is src:
-            raise ArgumentError("copycols cannot copy a panel matrix into itself")
+            raise ArgumentError("copycols needs distinct source and destination matrices")
         scols = _as_range(scols)
         _assert_column_range(scols, src.k, "source")
         if src.N != dest.N:
-            raise DimensionMismatch(f"the source has {src.N} rows and the destination has {dest.N}")
+            raise DimensionMismatch(f"row counts differ: {src.N} in the source, {dest.N} in the destination")
         if len(dcols) != len(scols):
-            raise DimensionMismatch(f"columns {scols} of the source and columns {dcols} of the destination differ in count")
+            raise DimensionMismatch(f"{len(scols)} source columns ({scols}) cannot fill {len(dcols)} destination "
+                                    f"columns ({dcols})")
         dense = np.empty((src.N, len(scols)), dtype=src.dtype, order="F")
         _collect_columns(src, scols, dense)
         return copycols(dest, dcols, dense)
@@ -572,7 +595,7 @@ def copyto(dest, src):
     """Whole-matrix copy (src/panels.jl:315-320)."""
     if isinstance(src, PanelMatrix):
         if (src.N, src.k) != (dest.N, dest.k):
-            raise DimensionMismatch(f"the source is {src.N}×{src.k} and the destination is {dest.N}×{dest.k}")
+            raise DimensionMismatch(f"copyto needs equal shapes; {src.N}×{src.k} into {dest.N}×{dest.k}")
         return copycols(dest, range(dest.k), src, range(src.k))
     return copycols(dest, range(dest.k), src)
 
@@ -618,7 +641,8 @@ class pooled_like:
 
 
 def similar(pm, dtype=None, dims=None):
-    """src/panels.jl:322-344: the width carries over, clamped to the new k."""
+    """src/panels.jl:322-344: a new matrix on the same plan, keeping the panel
+    width unless the new k is smaller."""
     T = pm.dtype if dtype is None else np.dtype(dtype)
     N, k = (pm.N, pm.k) if dims is None else (int(dims[0]), int(dims[1]))
     return PanelMatrix(N, k, plan=pm.plan, w=min(pm.w, k), dtype=T)
@@ -634,7 +658,12 @@ def _collect_columns(pm, cols, out):
             continue
         panel = pm.panels[j]
         dst = out[:, lo - cols.start:hi - cols.start]
-        if isinstance(panel.tier, GhostTier):
+        if isinstance(panel.tier, GhostTier) and panel.home.source.generator is not None:
+            from .ghosts import generate_panel
+            host = np.empty((pm.N, panel.width), dtype=host_storage_eltype(pm.plan, pm.dtype), order="F")
+            generate_panel(panel.home, host)
+            dst[...] = host[:, lo - pr.start:hi - pr.start]
+        elif isinstance(panel.tier, GhostTier):
             tmp = backend.alloc_device(pm.dtype, (pm.N, hi - lo))
             # The fill and the blocking copy back share the calling thread's
             # stream, so the copy sees the fill without a wait of its own.

@@ -86,16 +86,16 @@ class ResidencyPlan:
         if work < 0:
             raise ArgumentError(f"workspace_bytes must be nonnegative, got {work}")
         if nbuffers < 1:
-            raise ArgumentError(f"nbuffers must be at least 1, got {nbuffers}")
+            raise ArgumentError(f"a plan needs one staging buffer or more per operand; nbuffers = {nbuffers}")
         if target_panel_bytes < 1:
             raise ArgumentError(f"target_panel_bytes must be positive, got {target_panel_bytes}")
         if host <= 0:
             raise ArgumentError(f"host_budget must be positive, got {host}")
         if not dev > work:
             raise ArgumentError(
-                f"device_budget of {human_readable_bytes(dev)} does not cover the operator workspace of "
-                f"{human_readable_bytes(work)}, leaving nothing for panel buffers. Raise device_budget or "
-                "report a smaller workspace_bytes")
+                f"the operator workspace ({human_readable_bytes(work)}) takes the whole device_budget "
+                f"({human_readable_bytes(dev)}) and leaves no room for staging buffers; raise device_budget or "
+                "lower workspace_bytes")
         width = None if panel_width is None else int(panel_width)
         if width is not None and width < 1:
             raise ArgumentError(f"panel_width must be positive, got {width}")
@@ -105,7 +105,7 @@ class ResidencyPlan:
             try:
                 np.dtype(S)
             except TypeError:
-                raise ArgumentError(f"{name} must be a concrete type, got {S}")
+                raise ArgumentError(f"{name} has to name a numeric eltype; {S!r} does not")
             if not backend.supports_eltype(S):
                 raise ArgumentError(f"{name} {S} is not an eltype this backend stores")
         self.backend = backend
@@ -149,33 +149,34 @@ def device_bytes_required(N, w, T, nbuffers=2, extra_buffers=0, workspace_bytes=
 
 def choose_panel_width(N, k, T, *, device_budget, nbuffers=2, extra_buffers=0, workspace_bytes=0,
                        target_panel_bytes=2 * GiB):
-    """Largest width whose nbuffers + extra_buffers panel buffers and the
-    workspace fit the budget, capped at the target, then evened out over the
-    panels it implies (src/plan.jl:103-123)."""
+    """The panel width: the widest for which nbuffers + extra_buffers panel
+    buffers plus the workspace stay within the device budget, no wider than
+    target_panel_bytes allows, and then balanced so the panels it gives are
+    as equal as possible (src/plan.jl:103-123)."""
     if N <= 0:
         raise ArgumentError(f"N must be positive, got {N}")
     if k <= 0:
         raise ArgumentError(f"k must be positive, got {k}")
     buffers = int(nbuffers) + int(extra_buffers)
     if buffers < 1:
-        raise ArgumentError(f"nbuffers + extra_buffers must be at least 1, got {buffers}")
+        raise ArgumentError(f"a width is chosen for one buffer or more; nbuffers + extra_buffers = {buffers}")
     column = int(N) * np.dtype(T).itemsize
     available = int(np.floor(device_budget)) - int(np.floor(workspace_bytes))
     per_buffer = fld(available, buffers)
     if per_buffer < column:
         raise ArgumentError(
-            f"device budget of {human_readable_bytes(int(np.floor(device_budget)))} leaves "
-            f"{human_readable_bytes(available)} after a {human_readable_bytes(int(np.floor(workspace_bytes)))} "
-            f"operator workspace, so {human_readable_bytes(per_buffer)} for each of {buffers} panel buffers, "
-            f"but a single column of {N} {np.dtype(T)} entries needs {human_readable_bytes(column)}. Raise "
-            "device_budget, lower nbuffers, or narrow the compute eltype")
+            f"one column of {N} {np.dtype(T)} entries ({human_readable_bytes(column)}) does not fit a staging "
+            f"buffer: the device_budget of {human_readable_bytes(int(np.floor(device_budget)))}, less "
+            f"{human_readable_bytes(int(np.floor(workspace_bytes)))} of operator workspace, split over {buffers} "
+            f"buffers gives {human_readable_bytes(per_buffer)} each. Raise device_budget, use fewer buffers, or a "
+            "narrower compute eltype")
     w = min(int(k), max(1, fld(min(int(np.floor(target_panel_bytes)), per_buffer), column)))
     return cld(int(k), cld(int(k), w))
 
 
 def row_block_height(N, k, w):
-    """Rows of one block of the row traversal: an nrows x k block holds as much
-    as an N x w panel (src/plan.jl:125-133)."""
+    """Height of a block of the row traversal, chosen so that a block of all k
+    columns is about the size of one N x w panel (src/plan.jl:125-133)."""
     return min(max(fld(int(N) * int(w), int(k)), 1), int(N))
 
 
@@ -188,7 +189,7 @@ def resolve_panel_width(plan, N, k, T, override=None, extra_buffers=0):
                                   target_panel_bytes=plan.target_panel_bytes)
     w = int(requested)
     if not (1 <= w <= k):
-        raise ArgumentError(f"panel width {w} must lie in 1:{k}")
+        raise ArgumentError(f"a panel width for k = {k} lies between 1 and {k}; got {w}")
     needed = device_bytes_required(N, w, T, nbuffers=plan.nbuffers, extra_buffers=extra_buffers,
                                    workspace_bytes=plan.workspace_bytes)
     if needed > plan.device_budget:
@@ -197,10 +198,10 @@ def resolve_panel_width(plan, N, k, T, override=None, extra_buffers=0):
                                   extra_buffers=extra_buffers, workspace_bytes=plan.workspace_bytes,
                                   target_panel_bytes=plan.device_budget)
         raise ArgumentError(
-            f"panel width {w} needs {human_readable_bytes(needed)} on the device ({buffers} buffers of "
-            f"{human_readable_bytes(panel_bytes(N, w, T))} plus {human_readable_bytes(plan.workspace_bytes)} of "
-            f"operator workspace) but device_budget is {human_readable_bytes(plan.device_budget)}. Use "
-            f"panel_width ≤ {fits} or raise device_budget")
+            f"at panel width {w} the device would hold {buffers} buffers of "
+            f"{human_readable_bytes(panel_bytes(N, w, T))} and {human_readable_bytes(plan.workspace_bytes)} of "
+            f"operator workspace, {human_readable_bytes(needed)} in all, over the device_budget of "
+            f"{human_readable_bytes(plan.device_budget)}: choose panel_width ≤ {fits}, or a larger budget")
     return w
 
 
@@ -219,11 +220,11 @@ def checkout_device_buffers(plan, T, dims, count):
         if not cache:
             if pool.allocated + nbytes > pool.budget:
                 raise ArgumentError(
-                    f"device pool is full: a {dims[0]}×{dims[1]} {T} buffer needs {human_readable_bytes(nbytes)}, "
-                    f"{human_readable_bytes(pool.allocated)} is already allocated and the pool caps at "
-                    f"{human_readable_bytes(pool.budget)} (device_budget {human_readable_bytes(plan.device_budget)} "
-                    f"minus {human_readable_bytes(plan.workspace_bytes)} of operator workspace). Reduce the panel "
-                    "width, lower nbuffers, or raise device_budget")
+                    f"no room in the device pool for a {dims[0]}×{dims[1]} {T} buffer "
+                    f"({human_readable_bytes(nbytes)}): {human_readable_bytes(pool.allocated)} of its "
+                    f"{human_readable_bytes(pool.budget)} is in use (the pool is the device_budget of "
+                    f"{human_readable_bytes(plan.device_budget)} less the operator workspace). Use narrower "
+                    "panels, fewer buffers or a larger device_budget")
             buffers.append(plan.backend.alloc_device(T, dims))
             pool.allocated += nbytes
         else:
@@ -327,19 +328,20 @@ def host_full_error(plan, nbytes):
     remaining = pool.budget - pool.reserved
     stranded = host_bytes_free(pool)
     detail = "" if stranded == 0 else (
-        f", and the {human_readable_bytes(stranded)} already released is only reusable by blocks of the size "
-        "it was released at")
+        f"; another {human_readable_bytes(stranded)} is free but held in blocks of other sizes, which only "
+        "blocks of those sizes can reuse")
     if plan.scratch_dir is None:
         return ArgumentError(
-            f"host tier is full: a {human_readable_bytes(nbytes)} panel block does not fit in the "
-            f"{human_readable_bytes(remaining)} left of the {human_readable_bytes(pool.budget)} host_budget{detail}. "
-            "Raise host_budget, reduce k or the panel width, or give the plan a scratch_dir so panels can live on disk")
+            f"out of host memory: a panel block of {human_readable_bytes(nbytes)} exceeds the "
+            f"{human_readable_bytes(remaining)} still unreserved of the {human_readable_bytes(pool.budget)} "
+            f"host_budget{detail}. Raise host_budget, make k or the panels smaller, or set a scratch_dir so that "
+            "panels can move to disk")
     return ArgumentError(
-        f"host tier is full and no panel of that size is free to spill: a {human_readable_bytes(nbytes)} panel "
-        f"block does not fit in the {human_readable_bytes(remaining)} left of the "
-        f"{human_readable_bytes(pool.budget)} host_budget{detail}. A panel sweep holds up to {FEED_DEPTH + 2} "
-        "steps at once, one panel per matrix per step, and gram and cholqr2 traverse rows and so hold every "
-        "panel of their matrix. Raise host_budget to cover that many panels")
+        f"out of host memory, and every resident panel of this size is pinned, so no panel of that size is free "
+        f"to spill: a block of {human_readable_bytes(nbytes)} exceeds the {human_readable_bytes(remaining)} "
+        f"still unreserved of the {human_readable_bytes(pool.budget)} host_budget{detail}. A panel sweep pins the "
+        f"panels of up to {FEED_DEPTH + 2} steps (one per matrix per step), and a row sweep (gram, cholqr2) pins "
+        "every panel of its matrix; the host_budget has to cover that many")
 
 
 def register_resident(plan, panel):

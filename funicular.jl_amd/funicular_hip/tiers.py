@@ -16,7 +16,8 @@ class DiskTier(Tier):
 
 
 class GhostTier(Tier):
-    """A panel whose contents are a function of its index (src/tiers.jl:8-11)."""
+    """A panel that is computed from its column indices whenever it is needed
+    instead of being kept (src/tiers.jl:8-11)."""
 
     def __repr__(self):
         return "ghost"
@@ -89,16 +90,19 @@ def storage(handle):
 
 
 class GhostSource:
-    """Generator seed and shape shared by every panel of a ghost matrix
-    (src/tiers.jl:76-82). The generator itself is the device fill."""
+    """Generator, seed and shape shared by every panel of a ghost matrix
+    (src/tiers.jl:76-82). generator None is the device fill; otherwise it is
+    a host function called per column as generator(dst, rng, col)."""
 
-    __slots__ = ("seed", "N", "k", "w")
+    __slots__ = ("seed", "N", "k", "w", "generator", "dtype")
 
-    def __init__(self, seed, N, k, w):
+    def __init__(self, seed, N, k, w, generator=None, dtype=None):
         self.seed = seed
         self.N = N
         self.k = k
         self.w = w
+        self.generator = generator
+        self.dtype = dtype
 
     def __repr__(self):
         return f"GhostSource(seed = {self.seed:#x}, {self.N}×{self.k}, w = {self.w})"

@@ -146,7 +146,7 @@ end
 
 # A byte copy between two arrays of one eltype, queued on `stream`.
 function raw_copy!(dst, src, kind, stream)
-    size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    size(dst) == size(src) || throw(DimensionMismatch("sizes differ: destination $(size(dst)), source $(size(src))"))
     eltype(dst) === eltype(src) ||
         throw(ArgumentError("a byte copy needs one eltype on both sides, got $(eltype(src)) into $(eltype(dst)); " *
                             "h2d! and d2h! convert through a device staging buffer"))
@@ -172,7 +172,7 @@ end
 # allocates one on the device, moves the bytes, converts with the library's
 # kernel on the same stream, and waits before the buffer is let go.
 function converting_copy!(dst, src, b::HIPBackend, queue)
-    size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    size(dst) == size(src) || throw(DimensionMismatch("sizes differ: destination $(size(dst)), source $(size(src))"))
     stream = queue === nothing ? C_NULL : queue.stream
     staging = nothing
     task_local_storage(:fhip_stream, stream) do
@@ -292,7 +292,7 @@ function rightmul_gemm!(::HIPBackend, dst::HIPMatrix{T}, A::HIPMatrix{T}, C::HIP
 end
 
 function convert_copy!(::HIPBackend, dst::HIPMatrix{S}, src::HIPMatrix{T}) where {S,T}
-    size(dst) == size(src) || throw(DimensionMismatch("copy between a $(size(dst)) destination and a $(size(src)) source"))
+    size(dst) == size(src) || throw(DimensionMismatch("sizes differ: destination $(size(dst)), source $(size(src))"))
     check(ccall((:fhip_convert_copy, LIB[]), Cint,
         (Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}, Cint, Int64, Int64, Int64, Ptr{Cvoid}),
         dst.ptr, eltcode(S), dst.ld, src.ptr, eltcode(T), src.ld, size(dst, 1), size(dst, 2), current_stream()))

@@ -206,7 +206,7 @@ class JitteryBackend(DeviceBackend):
             src = src.reshape(-1, 1)
         if tuple(dst.shape) != tuple(src.shape):
             from funicular_hip.device_api import DimensionMismatch
-            raise DimensionMismatch(f"copy between a {dst.shape} destination and a {src.shape} source")
+            raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
         return self._launch(lambda: np.copyto(_np(dst), src.astype(dst.dtype, copy=False)), queue)
 
     def d2h(self, dst, src, queue=None):
@@ -214,7 +214,7 @@ class JitteryBackend(DeviceBackend):
             dst = dst.reshape(-1, 1)
         if tuple(dst.shape) != tuple(src.shape):
             from funicular_hip.device_api import DimensionMismatch
-            raise DimensionMismatch(f"copy between a {dst.shape} destination and a {src.shape} source")
+            raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
         return self._launch(lambda: np.copyto(dst, _np(src).astype(dst.dtype, copy=False)), queue)
 
     def d2d(self, dst, src, queue=None):

@@ -50,7 +50,7 @@ def test_panelmul_streams_a_disk_backed_pair(backend, tmp_path):
     N, k, w = 200, 24, 4
     A = randmatrix(rng, N, k)
     Gh = randmatrix(rng, N, N)
-    # Eight panels of room for the twelve the two matrices hold between them.
+    # The host budget holds 8 panels; X and Y have 6 each.
     plan = make_plan(backend, tmp_path, 8, N, w)
     X = F.PanelMatrix.from_array(A, plan=plan, w=w)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
@@ -64,8 +64,8 @@ def test_row_sweeps_over_disk_backed_matrices(backend, tmp_path):
     rng = np.random.default_rng(72)
     N, k, w = 400, 24, 4
     A, B = randmatrix(rng, N, k), randmatrix(rng, N, k)
-    # A row block is a slice of every panel, so gram and cholqr2 hold the whole
-    # matrix; seven panels cover one of the two matrices, not both.
+    # Row sweeps (gram, cholqr2) pin all 6 panels of a matrix at once; a
+    # budget of 7 fits that for X or for Y, never for both together.
     plan = make_plan(backend, tmp_path, 7, N, w)
     X = F.PanelMatrix.from_array(A, plan=plan, w=w)
     Y = F.PanelMatrix.from_array(B, plan=plan, w=w)

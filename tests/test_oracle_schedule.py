@@ -18,8 +18,8 @@ def _covers_rows_once(record, N):
 
 @pytest.mark.parametrize("N,k,w", SHAPES)
 def test_cholqr2_is_four_sweeps_of_the_whole_matrix(N, k, w):
-    # src/panelblas.jl:111: "Two passes cost four sweeps of the whole
-    # matrix"; a shifted pass adds a third pass (:114-120, :135-138).
+    # src/panelblas.jl:111 counts four full sweeps for the two passes; the
+    # shift adds a third pass (:114-120, :135-138).
     for shifted, passes in ((False, 2), (True, 3)):
         ev = OS.cholqr2(N, k, w, shifted=shifted)
         sw = OS.sweeps(ev)

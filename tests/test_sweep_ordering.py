@@ -75,7 +75,13 @@ def _operations():
         d = F.dot(X, Y, nbuffers=nb)
         return np.concatenate([F.to_matrix(Y).ravel(), [d]])
 
-    return [("gram", gram), ("gram_xy", gram2), ("cholqr2_reference", cholqr2("reference")),
+    def ghost_host(plan, backend, nb):
+        O = F.GhostPanels(F.randn_column, np.complex128, N, K, plan=plan, seed=9, w=W)
+        Y = F.PanelMatrix(N, K, plan=plan, w=W)
+        F.panelmul(Y, F.DenseOperator(G, backend), O, nbuffers=nb)
+        return np.concatenate([F.to_matrix(Y), F.gram(O, nbuffers=nb)])
+
+    return [("ghost_host", ghost_host), ("gram", gram), ("gram_xy", gram2), ("cholqr2_reference", cholqr2("reference")),
             ("cholqr2_fused", cholqr2("fused")), ("panelmul", panelmul), ("project_reference", project("reference")),
             ("project_runs", project("fused", group=2)), ("scale_axpy_dot", scale_dot)]
 
