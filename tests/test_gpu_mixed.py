@@ -88,11 +88,13 @@ def test_mixed_workload_at_its_configured_size(backend):
     V^H, 8,388,608 x 256 in panels of 16, resident in HBM, U the Q of cholqr2
     of a ghost basis (seed 0xdeadbeef), V a seeded 256 x 256 unitary, sigma
     log-spaced to 1e-2 and to 1e-4. The mixed first pass (FP32 Gram,
-    Cholesky and solve, or the solve in FP64) is held to the FP64-only run:
-    ||Q^H Q - I||_F <= 1e-13 for all three, ||Y - Q R|| / ||Y|| <= 1e-13 with
-    the FP64 solves and <= 1e-7 kappa with the FP32 one, and R within the
-    conditioning of the FP64-only R."""
-    from funicular_hip.executor import readpanels, sweep, updatepanels, writepanels
+    Cholesky and solve, or the solve in FP64), with the adaptive shift and
+    with the stated shift 11 (N k + k (k+1)) u32 ||Y||_2^2 always applied, is
+    held to the FP64-only run: ||Q^H Q - I||_F <= 1e-13 for all, ||Y - Q R|| /
+    ||Y|| <= 1e-13 with the FP64 solves and <= 1e-7 kappa with the FP32 one,
+    and R within the conditioning of the FP64-only R. The always-shifted
+    runs take three passes (six row sweeps), the others two."""
+    from funicular_hip.executor import readpanels, sweep, trace_sweeps, updatepanels, writepanels
     print("full size: the 8388608 x 256 mixed workload", flush=True)
     N, k, w = 8388608, 256, 16
     plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=1 << 30,
@@ -117,17 +119,23 @@ def test_mixed_workload_at_its_configured_size(backend):
             F.rightmul(Y0, U, sigma[:, None] * V.conj().T)
             results = {}
             for name, fn in (("fp64", F.cholqr2), ("mixed", cholqr2_mixed),
-                             ("mixed_fp64_solve", lambda X: cholqr2_mixed(X, solve="fp64"))):
+                             ("mixed_fp64_solve", lambda X: cholqr2_mixed(X, solve="fp64")),
+                             ("fukaya", lambda X: cholqr2_mixed(X, shift="fukaya")),
+                             ("fukaya_fp64_solve", lambda X: cholqr2_mixed(X, shift="fukaya", solve="fp64"))):
                 sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(Y0), writepanels(Y))
-                R = fn(Y)
+                with trace_sweeps() as sweeps:
+                    R = fn(Y)
                 orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
-                results[name] = (R, orth, resid(R))
+                results[name] = (R, orth, resid(R), len(sweeps))
             R64 = results["fp64"][0]
-            for name, (R, orth, res) in results.items():
-                print(f"sigma to 1e-{decades} {name}: orthogonality {orth:.2e}, residual {res:.2e}", flush=True)
+            for name, (R, orth, res, nsweeps) in results.items():
+                fp32_solve = name in ("mixed", "fukaya")
+                print(f"sigma to 1e-{decades} {name}: orthogonality {orth:.2e}, residual {res:.2e}, "
+                      f"{nsweeps} sweeps", flush=True)
+                assert nsweeps == (6 if name.startswith("fukaya") else 4), (name, nsweeps)
                 assert orth <= 1e-13, (name, orth)
-                assert res <= (1e-7 * kappa if name == "mixed" else 1e-13), (name, res)
-                tol = (1e-6 if name == "mixed" else 1e-14) * kappa
+                assert res <= (1e-7 * kappa if fp32_solve else 1e-13), (name, res)
+                tol = (1e-6 if fp32_solve else 1e-14) * kappa
                 assert np.linalg.norm(R - R64) <= tol * np.linalg.norm(R64), name
     finally:
         for pm in (U, Y0, Y, Z):
