@@ -137,6 +137,23 @@ int fhip_host_free(void *ptr) {
   return FHIP_OK;
 }
 
+int fhip_host_is_pinned(int *pinned, const void *ptr) {
+  if (!pinned) return fail(FHIP_E_ARGUMENT, "fhip_host_is_pinned: null output pointer");
+  *pinned = 0;
+  if (!ptr) return FHIP_OK;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, ptr);
+  if (e == hipErrorInvalidValue) {
+    // Pageable memory the runtime has never seen: not page-locked. The query
+    // leaves that error behind; clear it so the next call does not report it.
+    (void)hipGetLastError();
+    return FHIP_OK;
+  }
+  HIP_TRY(e, "hipPointerGetAttributes");
+  *pinned = attr.type == hipMemoryTypeHost ? 1 : 0;
+  return FHIP_OK;
+}
+
 int fhip_stream_create(void **stream) {
   if (!stream) return fail(FHIP_E_ARGUMENT, "fhip_stream_create: null output pointer");
   hipStream_t s;

@@ -64,6 +64,7 @@ _SIGNATURES = {
     "fhip_free": [_vp],
     "fhip_host_alloc": [ctypes.POINTER(_vp), _sz],
     "fhip_host_free": [_vp],
+    "fhip_host_is_pinned": [_vp, _vp],
     "fhip_stream_create": [ctypes.POINTER(_vp)],
     "fhip_stream_destroy": [_vp],
     "fhip_stream_sync": [_vp],

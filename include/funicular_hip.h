@@ -82,6 +82,11 @@ int fhip_free(void *ptr);
  * alignment of src/device_api.jl:140-154 is met by the allocator itself. */
 int fhip_host_alloc(void **ptr, size_t nbytes);
 int fhip_host_free(void *ptr);
+/* *pinned = 1 when ptr lies in page-locked host memory the runtime knows
+ * (an fhip_host_alloc block), 0 for pageable memory. The check the
+ * reference's CUDA suite makes with CUDA.is_pinned on a host slab
+ * (test/test_cuda.jl:24-36). */
+int fhip_host_is_pinned(int *pinned, const void *ptr);
 
 /* ---------------------------------------------------------------- queues -- */
 

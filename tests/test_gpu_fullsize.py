@@ -86,8 +86,8 @@ def test_sketch_chain_at_its_configured_size(backend):
     tridiagonal (2 + 0.1i, -1), cholqr2(Y), S = project(Q, G), against the
     oracle's chain on the same Omega (the ghost stream itself is bit-pinned to
     the oracle at smaller sizes). Bars: ||Q^H Q - I||_F <= 1e-13,
-    ||Y - Q R||_F / ||Y||_F <= 1e-13, S within 1e-12 normwise and entrywise
-    wherever |S_ij| >= 1e-3 max |S|."""
+    ||Y - Q R||_F / ||Y||_F <= 1e-13, S within 1e-12 normwise and in
+    every entry, relative to that entry."""
     from oracle import operators as OO
     from oracle import panelblas as OB
     print("full size: the 2^20 x 64 sketch chain", flush=True)
@@ -112,8 +112,12 @@ def test_sketch_chain_at_its_configured_size(backend):
     assert np.linalg.norm(Q @ R - Yo) / np.linalg.norm(Yo) <= 1e-13
     assert OB.isapprox(R, Ro, 1e-12)
     assert np.linalg.norm(S - So) <= 1e-12 * np.linalg.norm(So)
-    big = np.abs(So) >= 1e-3 * np.abs(So).max()
-    assert np.all(np.abs(S - So)[big] <= 1e-12 * np.abs(So)[big])
+    # Every entry, the smallest (|S_ij| ~ 3e-5 of max 3.3) included, within
+    # 1e-12 relative of the oracle's.
+    rel = np.abs(S - So) / np.abs(So)
+    print(f"S entrywise relative: max {rel.max():.2e}, |S_ij| from {np.abs(So).min():.2e} to "
+          f"{np.abs(So).max():.2e}", flush=True)
+    assert np.all(rel <= 1e-12)
     F.free(Y)
     del omega, plan
     gc.collect()
