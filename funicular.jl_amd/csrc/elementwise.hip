@@ -213,6 +213,139 @@ __global__ __launch_bounds__(256) void helmholtz_rows_kernel(T *__restrict__ Y, 
   store_elem<T, true>(Y + 2 * (int64_t)col * ldy, p, yr, yi);
 }
 
+// Shape of the blocked stencil: 256 E points of a plane per workgroup and L
+// planes. Two staged planes take 2 (256 E + 2 n1) complex entries of LDS,
+// 64 KiB at n1 = 512 in FP64, which is the largest n1 it takes.
+constexpr int HELM_E = 4;
+constexpr int HELM_L = 24;
+constexpr int64_t HELM_MAX_N1 = 512;
+
+// 2.5-D blocking of the 7-point stencil. A workgroup owns Q = 256 E
+// consecutive points q = i + n1 j of the plane and L consecutive planes l.
+// Each plane's points [q0 - n1, q0 + Q + n1) (the owned range and one i-row
+// either side, indices taken mod n1 n2, which is the j periodicity) are
+// staged in LDS, two planes at a time: the i and j neighbours of a point
+// come from its own plane's tile, its l + 1 neighbour from the next plane's
+// tile, and its l - 1 neighbour from registers (the centre value kept from
+// the plane before). The next-but-one plane's tile is loaded into registers
+// while the current plane is computed.
+template <typename T, int E, int L>
+__global__ __launch_bounds__(256) void helmholtz_tile_kernel(T *__restrict__ Y, int64_t ldy,
+                                                             const T *__restrict__ X, int64_t ldx, uint32_t n1,
+                                                             uint32_t n2, uint32_t n3, DivU32 d1, DivU32 dq,
+                                                             DivU32 dl, T dr, T di, T orr, T oi) {
+  typedef typename Pair<T>::type P2;
+  extern __shared__ __align__(16) unsigned char helm_lds[];
+  constexpr uint32_t Q = 256u * E;
+  const uint32_t s3 = n1 * n2;
+  const uint32_t span = Q + 2u * n1;  // points per staged plane
+  P2 *tile[2] = {reinterpret_cast<P2 *>(helm_lds), reinterpret_cast<P2 *>(helm_lds) + span};
+  uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const uint32_t r1 = divu32(b, dq);
+  const uint32_t qt = b - r1 * dq.d;
+  const uint32_t col = divu32(r1, dl);
+  const uint32_t lg = r1 - col * dl.d;
+  const uint32_t q0 = qt * Q;
+  const uint32_t l0 = lg * (uint32_t)L;
+  const uint32_t nl = n3 - l0 < (uint32_t)L ? n3 - l0 : (uint32_t)L;
+  const uint32_t t = threadIdx.x;
+  const P2 *x = reinterpret_cast<const P2 *>(X + 2 * (int64_t)col * ldx);
+  T *y = Y + 2 * (int64_t)col * ldy;
+  // Staging: thread t moves points h = t + 256 r of a plane's span.
+  constexpr int R = (int)((256u * E + 2u * 512u + 255u) / 256u);  // enough for n1 <= 512
+  const int nr = (int)((span + 255u) / 256u);
+  uint32_t gidx[R];
+  const uint32_t hbase = q0 + s3 - n1;  // h = 0 is plane point q0 - n1 (mod s3)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    uint32_t g = hbase + t + 256u * (uint32_t)r;
+    while (g >= s3) g -= s3;
+    gidx[r] = g;
+  }
+  P2 stage[R];
+  auto fetch = [&](uint32_t l) {
+    const P2 *xp = x + (int64_t)s3 * l;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (r < nr) stage[r] = xp[gidx[r]];
+  };
+  auto stash = [&](P2 *dst) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const uint32_t h = t + 256u * (uint32_t)r;
+      if (r < nr && h < span) dst[h] = stage[r];
+    }
+  };
+  // The thread's points and their local (tile) offsets of the i neighbours.
+  uint32_t hq[E];
+  int dim[E], dip[E];
+  bool live[E];
+  T pr[E], pi[E];
+  const uint32_t lprev = l0 == 0 ? n3 - 1 : l0 - 1;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t q = q0 + 256u * e + t;
+    live[e] = q < s3;
+    const uint32_t qc = live[e] ? q : s3 - 1;
+    const uint32_t j = divu32(qc, d1), i = qc - j * n1;
+    hq[e] = q - q0 + n1;
+    dim[e] = i == 0 ? (int)n1 - 1 : -1;
+    dip[e] = i == n1 - 1 ? 1 - (int)n1 : 1;
+    const P2 v = x[(int64_t)s3 * lprev + qc];
+    pr[e] = v.x;
+    pi[e] = v.y;
+  }
+  fetch(l0);
+  stash(tile[0]);
+  fetch(l0 + 1 == n3 ? 0 : l0 + 1);
+  stash(tile[1]);
+  if (nl > 1) {
+    uint32_t l2 = l0 + 2;
+    if (l2 >= n3) l2 -= n3;
+    fetch(l2);
+  }
+  __syncthreads();
+  for (uint32_t s = 0; s < nl; ++s) {
+    const P2 *cur = tile[s & 1];
+    const P2 *nxt = tile[(s & 1) ^ 1];
+    const uint32_t l = l0 + s;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t h = hq[e];
+      const P2 c = cur[h];
+      const P2 a = cur[h + dim[e]];
+      const P2 bb = cur[h + dip[e]];
+      const P2 jm = cur[h - n1];
+      const P2 jp = cur[h + n1];
+      const P2 lp = nxt[h];
+      T sr = a.x + bb.x, si = a.y + bb.y;
+      sr += jm.x;
+      si += jm.y;
+      sr += jp.x;
+      si += jp.y;
+      sr += pr[e];
+      si += pi[e];
+      sr += lp.x;
+      si += lp.y;
+      const T yr = dr * c.x - di * c.y + orr * sr - oi * si;
+      const T yi = dr * c.y + di * c.x + orr * si + oi * sr;
+      if (live[e]) store_elem<T, true>(y, (int64_t)s3 * l + (q0 + 256u * e + t), yr, yi);
+      pr[e] = c.x;
+      pi[e] = c.y;
+    }
+    if (s + 1 < nl) {
+      __syncthreads();
+      stash(tile[s & 1]);  // plane l + 2 replaces plane l
+      if (s + 2 < nl) {
+        uint32_t l3 = l + 3;
+        while (l3 >= n3) l3 -= n3;
+        fetch(l3);
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // Workgroups for the row-chunk stencils, or 0 when the sizes need the 64-bit
 // grid-stride forms.
 static uint32_t rows_grid(int64_t N, int64_t w, int64_t &chunks) {
@@ -551,6 +684,24 @@ extern "C" int fhip_stencil_helmholtz3d(void *Y, int64_t ldy, const void *X, int
   }
   hipStream_t s = (hipStream_t)stream;
   int64_t chunks;
+  const int64_t s3 = n1 * n2;
+  const int64_t qtiles = cdiv64(s3, 256 * HELM_E), lgroups = cdiv64(n3, HELM_L);
+  if (N < (int64_t(1) << 31) && s3 < (int64_t(1) << 30) && n1 <= HELM_MAX_N1 &&
+      qtiles * lgroups * w < (int64_t(1) << 31)) {
+    const DivU32 d1 = make_divu32((uint32_t)n1), dq = make_divu32((uint32_t)qtiles);
+    const DivU32 dl = make_divu32((uint32_t)lgroups);
+    const unsigned nb = (unsigned)(qtiles * lgroups * w);
+    const size_t span = 256 * HELM_E + 2 * n1;
+    if (type == FHIP_C64)
+      hipLaunchKernelGGL((helmholtz_tile_kernel<double, HELM_E, HELM_L>), dim3(nb), dim3(256), 2 * span * 16, s,
+                         (double *)Y, ldy, (const double *)X, ldx, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, d1,
+                         dq, dl, d_re, d_im, o_re, o_im);
+    else
+      hipLaunchKernelGGL((helmholtz_tile_kernel<float, HELM_E, HELM_L>), dim3(nb), dim3(256), 2 * span * 8, s,
+                         (float *)Y, ldy, (const float *)X, ldx, (uint32_t)n1, (uint32_t)n2, (uint32_t)n3, d1, dq,
+                         dl, (float)d_re, (float)d_im, (float)o_re, (float)o_im);
+    return check_launch("helmholtz_tile_kernel");
+  }
   if (const uint32_t nb = rows_grid(N, w, chunks)) {
     const DivU32 d1 = make_divu32((uint32_t)n1), d2 = make_divu32((uint32_t)n2);
     const DivU32 dc = make_divu32((uint32_t)chunks);

@@ -194,15 +194,22 @@ __device__ inline double gh_normal(uint64_t key, uint64_t d) {
 // The first attempt's fast accept alone: true and the normal in v when the
 // draw's first word lands inside its layer's rectangle, as gh_normal_from's
 // first iteration decides; false otherwise, when the full routine is needed.
-__device__ inline bool gh_normal_fast(const double *__restrict__ wt, const uint64_t *__restrict__ kt,
-                                      uint64_t key, uint64_t d, double &v) {
-  const uint64_t w0 = gh_word(key, 0, d);
+// The mix's argument z = key + GAMMA (d + 1) comes in already formed, and a
+// layer's edge and bound are read together.
+struct ZigPair {
+  double wt;
+  uint64_t kt;
+};
+
+__device__ inline bool gh_normal_fast_z(const ZigPair *__restrict__ tab, uint64_t z, double &v) {
+  const uint64_t w0 = gh_mix(z);
   const int i = (int)(w0 & 0xFF);
   const bool neg = (w0 >> 8) & 1;
   const uint64_t u = w0 >> 11;
-  const double x = (double)u * wt[i];
+  const ZigPair p = tab[i];
+  const double x = (double)u * p.wt;
   v = neg ? -x : x;
-  return u < kt[i];
+  return u < p.kt;
 }
 
 template <typename T, bool CPLX>
@@ -228,13 +235,16 @@ __global__ __launch_bounds__(256) void ghost_kernel(T *__restrict__ dst, int64_t
 // than 2^31 rows and workgroups. Each thread owns GH_E entries, GH_ROWS / GH_E
 // rows apart. It takes every draw's first fast accept and stores the entries;
 // a draw that misses it (about 1 in 100: a wedge or tail test, or a retry)
-// sets a bit, and the thread then resolves its own missed draws with the full
-// routine, storing that part again. A wave so runs the slow path a couple of
-// times per GH_E * 128 draws instead of for almost every 128, since with one
-// entry per thread some lane of the wave nearly always needs it. The values
-// are gh_normal's: the fast accept is its first test, and a missed draw is
-// recomputed by it from the first attempt. The column key is computed once per
-// thread and the ziggurat tables are read from an LDS copy.
+// goes into a workgroup queue in LDS, and after a barrier the threads resolve
+// the queued draws one each with the full routine, storing that part again.
+// (Resolved by their own threads, a wave would loop over the union of its
+// lanes' misses with one or two lanes active.) The values are gh_normal's:
+// the fast accept is its first test, and a missed draw is recomputed by it
+// from the first attempt. The column key is computed once per thread; the
+// mix's argument key + GAMMA (d + 1) is formed once and then stepped by
+// constant multiples of GAMMA (the same sums mod 2^64 as the products); the
+// ziggurat tables are read from an LDS copy, a layer's edge and bound as one
+// pair.
 constexpr int GH_E = 8;
 constexpr int GH_ROWS = 256 * GH_E;
 
@@ -242,50 +252,79 @@ template <typename T, bool CPLX>
 __global__ __launch_bounds__(256) void ghost_rows_kernel(T *__restrict__ dst, int64_t ld, int64_t row0,
                                                          uint32_t nrows, DivU32 chunks, int64_t col0,
                                                          uint64_t seed) {
+  constexpr int NP = CPLX ? 2 : 1;
   __shared__ double s_wt[256];
   __shared__ double s_f[257];
   __shared__ uint64_t s_kt[256];
+  __shared__ ZigPair s_tab[256];
+  // Queue entries: (draw bit) * 256 + thread; every draw of the workgroup
+  // fits, so no miss can overflow it.
+  __shared__ uint32_t s_queue[256 * GH_E * NP];
+  __shared__ uint32_t s_count;
   const int t = threadIdx.x;
   s_wt[t] = g_zig.wt[t];
   s_kt[t] = g_zig.kt[t];
+  s_tab[t].wt = g_zig.wt[t];
+  s_tab[t].kt = g_zig.kt[t];
   s_f[t] = g_zig.f[t];
-  if (t == 0) s_f[256] = g_zig.f[256];
+  if (t == 0) {
+    s_f[256] = g_zig.f[256];
+    s_count = 0;
+  }
   __syncthreads();
   const uint32_t b = blockIdx.x;
   const uint32_t c = divu32(b, chunks);
   const uint32_t base = (b - c * chunks.d) * (uint32_t)GH_ROWS;
   const uint64_t key = gh_colkey(seed, col0 + (int64_t)c);
-  constexpr int NP = CPLX ? 2 : 1;
   T *out = dst + NP * (int64_t)c * ld;
   const double scale = CPLX ? 0.70710678118654752 : 1.0;
+  // Draw d of row r is d = 2 r + part (complex) or r (real); its mix
+  // argument is key + GAMMA (d + 1). Entry j of this thread is row
+  // row0 + base + 256 j + t, so entry j + 1's argument is entry j's plus
+  // GAMMA * 256 NP.
+  const uint64_t first_row = (uint64_t)(row0 + (int64_t)(base + (uint32_t)t));
+  uint64_t z = key + GH_GAMMA * ((CPLX ? 2 * first_row : first_row) + 1);
+  const uint64_t zstep = GH_GAMMA * (uint64_t)(NP * 256);
   uint32_t pending = 0;
 #pragma unroll
   for (int j = 0; j < GH_E; ++j) {
     const uint32_t i = base + (uint32_t)(j * 256 + t);
+    const uint64_t zj = z;
+    z += zstep;
     if (i < nrows) {
-      const uint64_t row = (uint64_t)(row0 + (int64_t)i);
       if (CPLX) {
         double re, im;
-        const bool okr = gh_normal_fast(s_wt, s_kt, key, 2 * row, re);
-        const bool oki = gh_normal_fast(s_wt, s_kt, key, 2 * row + 1, im);
+        const bool okr = gh_normal_fast_z(s_tab, zj, re);
+        const bool oki = gh_normal_fast_z(s_tab, zj + GH_GAMMA, im);
         pending |= (okr ? 0u : 1u) << (2 * j);
         pending |= (oki ? 0u : 1u) << (2 * j + 1);
         store_elem<T, true>(out, i, (T)(re * scale), (T)(im * scale));
       } else {
         double v;
-        const bool ok = gh_normal_fast(s_wt, s_kt, key, row, v);
+        const bool ok = gh_normal_fast_z(s_tab, zj, v);
         pending |= (ok ? 0u : 1u) << j;
         store_elem<T, false>(out, i, (T)v, T(0));
       }
     }
   }
-  // At most NP * GH_E bits, one cleared per pass.
-  while (pending) {
-    const int bit = __builtin_ctz(pending);
-    pending &= pending - 1;
+  if (pending) {
+    uint32_t slot = atomicAdd(&s_count, (uint32_t)__builtin_popcount(pending));
+    while (pending) {
+      const int bit = __builtin_ctz(pending);
+      pending &= pending - 1;
+      s_queue[slot++] = (uint32_t)bit * 256u + (uint32_t)t;
+    }
+  }
+  // The barrier also orders each first store before the store that replaces
+  // one part of it.
+  __syncthreads();
+  const uint32_t nq = s_count;
+  for (uint32_t q = (uint32_t)t; q < nq; q += 256u) {
+    const uint32_t code = s_queue[q];
+    const int bit = (int)(code >> 8);
     const int j = CPLX ? bit >> 1 : bit;
     const int part = CPLX ? bit & 1 : 0;
-    const uint32_t i = base + (uint32_t)(j * 256 + t);
+    const uint32_t i = base + (uint32_t)(j * 256) + (code & 255u);
     const uint64_t row = (uint64_t)(row0 + (int64_t)i);
     const uint64_t d = CPLX ? 2 * row + (uint64_t)part : row;
     out[NP * (int64_t)i + part] = (T)(gh_normal_from(s_wt, s_f, s_kt, key, d) * scale);
