@@ -41,6 +41,11 @@ namespace fhip {
 #define FHIP_TRSM_KC_FP32 16
 #endif
 constexpr int TW_THREADS = 256;  // 4 waves, each a 32 x 32 tile of the block
+// The solve Y R^-1 (trsm_wide_kernel) runs 8 waves, each a 32 x 16 tile of
+// the block: 128 registers a lane, so two workgroups (four waves a SIMD)
+// share a CU, against two waves a SIMD for 32 x 32 tiles at 234 registers.
+constexpr int TWW_THREADS = 512;
+constexpr int TWW_WAVES_EU = TWW_THREADS == 256 ? 2 : 4;
 constexpr int TW_CB = FHIP_TRSM_CB;
 constexpr int TW_ROWS = 4096 / TW_CB;
 #ifndef FHIP_TRSM_SWIZZLE
@@ -430,21 +435,22 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 // M_JJ meets each 16-column tile of a wave as a zero block, a triangle or a
 // full block; the products with zero blocks are left out.
 template <typename TS, typename T, bool CPLX, bool CT>
-__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void trsm_wide_kernel(
+__global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW_WAVES_EU))) void trsm_wide_kernel(
     TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int TW_KC = TrsmSlice<T>::KC;
-  constexpr int TW_APT = TrsmSlice<T>::APT;
-  constexpr int TW_BPT = TrsmSlice<T>::BPT;
+  constexpr int TW_APT = TW_KC * TW_ROWS / TWW_THREADS;
+  constexpr int TW_BPT = TW_KC * TW_CB / TWW_THREADS;
+  constexpr int WNB = TWW_THREADS == 256 ? 2 : 1;  // 16-column tiles per wave
   constexpr int SWZ = !(CPLX && FHIP_TRSM_SWIZZLE) ? 0
                       : sizeof(T) == 8                                     ? 1
                       : (TW_ROWS == 64 && TW_CB == 64 && TW_KC == 16)      ? 2
                                                                            : 0;
   constexpr int TW_LDA = SWZ ? TW_ROWS : TW_ROWS + 4;
   constexpr int TW_LDB = SWZ ? TW_CB : TW_CB + 4;
-  constexpr int YCS = TW_THREADS / TW_ROWS;
-  constexpr int MCS = TW_THREADS / TW_KC;
-  static_assert(TW_THREADS % TW_ROWS == 0 && TW_THREADS % TW_KC == 0, "staging layout");
+  constexpr int YCS = TWW_THREADS / TW_ROWS;
+  constexpr int MCS = TWW_THREADS / TW_KC;
+  static_assert(TWW_THREADS % TW_ROWS == 0 && TWW_THREADS % TW_KC == 0, "staging layout");
   static_assert(YCS * TW_APT == TW_KC && MCS * TW_BPT == TW_CB, "staging covers the slice");
   __shared__ T As[NP][TW_KC * TW_LDA];
   __shared__ T Bs[NP][TW_KC * TW_LDB];
@@ -460,7 +466,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
   const int lk = lane >> 4;
   constexpr int WAVE_ROWS = TW_ROWS / 32;
   const int mi0 = 32 * (w % WAVE_ROWS);
-  const int nj0 = 32 * (w / WAVE_ROWS);
+  const int nj0 = 16 * WNB * (w / WAVE_ROWS);
   const int nbk = (k + TW_CB - 1) / TW_CB;
 
   const int yr = t % TW_ROWS, yk = t / TW_ROWS;
@@ -522,11 +528,11 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     }
   };
 
-  acc_t p1[2][2], p2[2][2], p3[2][2];
+  acc_t p1[2][WNB], p2[2][WNB], p3[2][WNB];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
+    for (int nb = 0; nb < WNB; ++nb) {
       p1[mb][nb] = Mfma<T>::zero();
       p2[mb][nb] = Mfma<T>::zero();
       p3[mb][nb] = Mfma<T>::zero();
@@ -583,29 +589,33 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const bool last = cur.idx + 1 >= cur.n;
     const int c0 = cur.c0;
     bool contributes = c0 + nj0 < k && !(cur.idx >= cur.real && cur.idx < cur.own);
-    unsigned nb_mask = 3u;
+    constexpr unsigned FULL = (1u << WNB) - 1u;
+    unsigned nb_mask = FULL;
     if (cur.idx < cur.own) {
       const int kl = cur.kc - c0;
-      contributes = contributes && (CT ? (kl + TW_KC > nj0) : (kl < nj0 + 32));
+      contributes = contributes && (CT ? (kl + TW_KC > nj0) : (kl < nj0 + 16 * WNB));
       if constexpr (TW_KC == 16) {
         const int rb = kl >> 4, cb0 = nj0 >> 4;
-        nb_mask = CT ? ((rb >= cb0 ? 1u : 0u) | (rb >= cb0 + 1 ? 2u : 0u))
-                     : ((rb <= cb0 ? 1u : 0u) | (rb <= cb0 + 1 ? 2u : 0u));
+        if constexpr (WNB == 2)
+          nb_mask = CT ? ((rb >= cb0 ? 1u : 0u) | (rb >= cb0 + 1 ? 2u : 0u))
+                       : ((rb <= cb0 ? 1u : 0u) | (rb <= cb0 + 1 ? 2u : 0u));
+        else
+          nb_mask = CT ? (rb >= cb0 ? 1u : 0u) : (rb <= cb0 ? 1u : 0u);
       }
     }
-    if (contributes && nb_mask == 3u) {
+    if (contributes && nb_mask == FULL) {
       if constexpr (CPLX)
-        mma_steps_3m<T, 2, 2, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
+        mma_steps_3m<T, 2, WNB, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
                                              TW_KC / 4, 1, p1, p2, p3);
       else
-        mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
+        mma_steps<T, false, 2, WNB, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
                                             1, p1, p2);
     } else if (contributes) {
       if constexpr (CPLX)
-        mma_steps_3m<T, 2, 2, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
+        mma_steps_3m<T, 2, WNB, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
                                              TW_KC / 4, 1, p1, p2, p3, nb_mask);
       else
-        mma_steps<T, false, 2, 2, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
+        mma_steps<T, false, 2, WNB, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
                                             1, p1, p2);
     }
     if (!last) {
@@ -615,7 +625,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-      for (int nb = 0; nb < 2; ++nb) {
+      for (int nb = 0; nb < WNB; ++nb) {
         acc_t re = p1[mb][nb], im = p2[mb][nb];
         if constexpr (CPLX) combine3m<PROD_ADD>(p1[mb][nb], p2[mb][nb], p3[mb][nb], re, im);
 #pragma unroll
@@ -646,7 +656,7 @@ static int launch_wide(TS *Y, int64_t ldy, int64_t m, int k, const T *M, hipStre
     hipLaunchKernelGGL((trsm_wide_short_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream,
                        Y, ldy, m, k, M, kp);
   else
-    hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream, Y,
+    hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TWW_THREADS), 0, stream, Y,
                        ldy, m, k, M, kp);
   return check_launch("trsm_wide_kernel");
 }
