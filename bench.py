@@ -531,13 +531,15 @@ def range_finder_bench(backend, scratch, n=160, k=256, w=32, iterations=2):
     then `iterations` power steps of (Z = G^H Y, orthonormalize Z, Y = G Z,
     orthonormalize Y), every orthonormalization cholqr2_panelwise with the
     mixed first pass. Y and Z live on the scratch disk behind a host budget of
-    4 of their 8 panels, so the row sweeps of cholqr2 could not run at all.
-    Reports the time of each step, the disk traffic and ||Q^H Q - I||_F."""
+    2 (FEED_DEPTH + 2) = 8 panels (what a panel sweep over two matrices may pin
+    at once) against their 16, so the row sweeps of cholqr2 could not run at
+    all. Reports the time of each step, the disk traffic and ||Q^H Q - I||_F."""
     import funicular_hip as F
+    from funicular_hip.plan import FEED_DEPTH
     N = n ** 3
     panel = N * w * 16
-    plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=4 * panel + (1 << 20),
-                           scratch_dir=scratch, panel_width=w)
+    plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3,
+                           host_budget=2 * (FEED_DEPTH + 2) * panel + (1 << 20), scratch_dir=scratch, panel_width=w)
     d = 6.0 - 0.09 * (1 + 0.05j)
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
     Gh = G.adjoint()
@@ -574,7 +576,7 @@ def range_finder_bench(backend, scratch, n=160, k=256, w=32, iterations=2):
     read = sum(s["disk_read_GB"] for s in steps)
     written = sum(s["disk_write_GB"] for s in steps)
     out = {"workload": f"range finder on a {n}^3 Helmholtz grid: N={N}, k={k}, panels of {w} ({panel / 1e9:.2f} GB), "
-                       f"{iterations} power iterations, mixed first pass, host budget 4 of 8 panels",
+                       f"{iterations} power iterations, mixed first pass, host budget 8 of their 16 panels",
            "total_s": total, "steps": steps, "disk_read_GB": read, "disk_write_GB": written,
            "disk_GBps": (read + written) / total, "orthogonality": orth}
     log(f"range finder: {total:.1f} s, ||Q^H Q - I||_F = {orth:.2e}")
