@@ -305,7 +305,7 @@ def adjoint_bench(backend):
     G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, d, backend=backend)
 
     def ghost_copy(seed):
-        src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=seed, w=w)
+        src = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=seed, w=w)
         dst = F.PanelMatrix(N, k, plan=plan, w=w)
         sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(dst))
         return dst
@@ -401,7 +401,7 @@ def mixed_bench(backend, steps):
     plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=1 << 30,
                            device_resident=True)
     U = F.PanelMatrix(N, k, plan=plan, w=w)
-    src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    src = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(U))
     F.cholqr2(U)
     rng = np.random.default_rng(0xDEADBEEF + 1)
@@ -478,7 +478,7 @@ def disk_bench(backend, scratch):
     for label, budget, sdir in (("disk", 4 * panel + (1 << 20), scratch), ("host", 9 * panel, None)):
         plan = F.ResidencyPlan(backend=backend, device_budget=64 * 1024 ** 3, host_budget=budget,
                                scratch_dir=sdir)
-        omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+        omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
         Y = F.PanelMatrix(N, k, plan=plan, w=w)
         C = backend.alloc_device(np.complex128, (w, w))
         backend.sync_all()
@@ -543,7 +543,7 @@ def range_finder_bench(backend, scratch, n=160, k=256, w=32, iterations=2):
     d = 6.0 - 0.09 * (1 + 0.05j)
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
     Gh = G.adjoint()
-    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
     Z = F.PanelMatrix(N, k, plan=plan, w=w)
     steps = []
@@ -870,7 +870,7 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
     log(f"streamed: {N}x{k} host-resident basis allocated and page-locked in {time.perf_counter() - t:.1f} s")
     d = 6.0 - 0.09 * (1 + 0.05j)
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
-    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     c0 = F._lib.counters()
     t = time.perf_counter()
     F.panelmul(Y, G, omega)
@@ -972,7 +972,7 @@ def streamed_narrow(args):
     log(f"streamed-narrow: {N}x{k} ComplexF32 host basis allocated in {time.perf_counter() - t:.1f} s")
     d = 6.0 - 0.09 * (1 + 0.05j)
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
-    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     t = time.perf_counter()
     F.panelmul(Y, G, omega)
     backend.sync_all()
@@ -1074,7 +1074,7 @@ def main():
                            device_resident=True)
     d = 6.0 - 0.09 * (1 + 0.05j)
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
-    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
 
     def regenerate():

@@ -16,7 +16,7 @@ from .device_api import (ArgumentError, DeviceArray, DimensionMismatch, HIPBacke
 from .adjoint import rrule_cholqr2, rrule_project  # noqa: E402
 from .executor import (PanelSteps, RowSteps, foreachpanel, readpanels, rowtraversal, sweep,  # noqa: E402
                        updatepanels, writepanels)
-from .ghosts import GhostPanels, randn_column  # noqa: E402
+from .ghosts import GhostPanels, device_randn, randn_column  # noqa: E402
 from .io import disk_reads, disk_writes, load, save  # noqa: E402
 from .operator import (DenseOperator, Helmholtz3D, Tridiagonal1D, check_operator, ishermitian_op,  # noqa: E402
                        panel_capable, workspace_bytes)
@@ -28,7 +28,7 @@ from .plan import (ResidencyPlan, choose_panel_width, device_bytes_allocated, re
                    row_block_height)
 
 __all__ = [
-    "GhostPanels", "randn_column", "PanelMatrix", "ResidencyPlan", "check_operator", "cholqr2", "copycols", "foreachpanel",
+    "GhostPanels", "device_randn", "randn_column", "PanelMatrix", "ResidencyPlan", "check_operator", "cholqr2", "copycols", "foreachpanel",
     "gram", "npanels", "panelmul", "panelrange", "panelwidth", "project", "rightmul", "hip_backend",
     "rrule_cholqr2", "rrule_project", "save", "load", "disk_reads", "disk_writes", "scale", "axpy", "dot", "norm",
     "cholqr2_panelwise", "gram_panelwise", "rdiv_panelwise",

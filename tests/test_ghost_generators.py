@@ -67,3 +67,15 @@ def test_a_ghost_with_a_host_generator_refuses_writes_and_bad_arguments():
         F.GhostPanels(np.complex128, 50, 6, plan=plan, generator=3)
     with pytest.raises(F.ArgumentError):
         F.GhostPanels(np.complex128, 0, 6, plan=plan)
+
+
+def test_the_default_generator_is_randn_column_and_the_device_stream_is_named():
+    # GhostPanels(T, N, k) takes the host generator, as the reference's takes
+    # randn! (src/ghosts.jl:53-63); the device stream is chosen by name.
+    plan = _plan()
+    default = F.to_matrix(F.GhostPanels(np.complex128, 120, 9, plan=plan, seed=5, w=4))
+    named = F.to_matrix(F.GhostPanels(F.randn_column, np.complex128, 120, 9, plan=plan, seed=5, w=4))
+    assert np.array_equal(default, named)
+    device = F.GhostPanels(F.device_randn, np.complex128, 120, 9, plan=plan, seed=5, w=4)
+    assert all(p.home.source.generator is None for p in device.panels)
+    assert repr(F.device_randn) == "device_randn"

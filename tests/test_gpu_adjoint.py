@@ -189,7 +189,7 @@ def test_adjoint_workload_at_its_configured_size(backend):
     G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, d, backend=backend)
 
     def ghost_copy(seed):
-        src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=seed, w=w)
+        src = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=seed, w=w)
         dst = F.PanelMatrix(N, k, plan=plan, w=w)
         sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(dst))
         return dst

@@ -121,7 +121,7 @@ def test_elementwise_rejects_mismatched_operands(backend):
 def test_ghosts_refuse_scale_and_axpy(backend):
     plan = make_plan(backend)
     N, k, w = 16, 6, 4
-    O = F.GhostPanels(np.complex128, N, k, plan=plan, w=w)
+    O = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, w=w)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
     with pytest.raises(F.ArgumentError):
         F.scale(O, 2)
@@ -139,7 +139,7 @@ def test_reductions_at_a_large_ghost_basis(backend):
     N, k, w = 1 << 22, 32, 16
     print(f"large reductions N={N} k={k}", flush=True)
     plan = F.ResidencyPlan(backend=backend, device_budget=16 * 2 ** 30, host_budget=16 * 2 ** 30)
-    O = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    O = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     X = F.PanelMatrix(N, k, plan=plan, w=w)
     F.copyto(X, O)
     n = F.norm(X)

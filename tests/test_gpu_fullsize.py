@@ -23,7 +23,7 @@ def basis(backend):
     plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=8 * 1024 ** 3,
                            device_resident=True)
     G = F.Helmholtz3D(N_EDGE, N_EDGE, N_EDGE, D, -1.0, backend=backend)
-    omega = F.GhostPanels(np.complex128, N, K, plan=plan, seed=SEED, w=W)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, K, plan=plan, seed=SEED, w=W)
     Y = F.PanelMatrix(N, K, plan=plan, w=W)
     yield plan, G, omega, Y
     F.free(Y)
@@ -94,7 +94,7 @@ def test_sketch_chain_at_its_configured_size(backend):
     N, k, w = 2 ** 20, 64, 8
     d, o = 2 + 0.1j, -1.0
     plan = F.ResidencyPlan(backend=backend, device_budget=16 * 1024 ** 3, host_budget=8 * 1024 ** 3)
-    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=SEED, w=w)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=SEED, w=w)
     G = F.Tridiagonal1D(N, d, o, backend=backend)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
     F.panelmul(Y, G, omega)

@@ -30,5 +30,5 @@ def test_host_generated_ghost_panels_feed_the_sweeps(backend, host_eltype):
     assert np.allclose(F.to_matrix(Y), OO.tridiag_apply(A, d, o), rtol=tol, atol=tol)
     assert OB.isapprox(F.gram(O), A.conj().T @ A, OB.blastol(np.complex128, N, k))
     # The device stream and the host generator are different streams.
-    D = F.to_matrix(F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w))
+    D = F.to_matrix(F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w))
     assert not np.array_equal(D, A)

@@ -100,7 +100,7 @@ def test_mixed_workload_at_its_configured_size(backend):
     plan = F.ResidencyPlan(backend=backend, device_budget=200 * 1024 ** 3, host_budget=1 << 30,
                            device_resident=True)
     U = F.PanelMatrix(N, k, plan=plan, w=w)
-    src = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    src = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     sweep(lambda _, a, b: backend.convert_copy(b, a), readpanels(src), writepanels(U))
     F.cholqr2(U)
     rng = np.random.default_rng(0xDEADBEEF + 1)

@@ -160,11 +160,11 @@ def test_rightmul_same_bits_at_nbuffers(backend):
 def test_ghost_regenerates_and_matches_oracle(backend, w):
     N, k = 24, 8
     plan = make_plan(backend)
-    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    O = F.GhostPanels(F.device_randn, C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     A = F.to_matrix(O)
     assert np.array_equal(A, OG.matrix(C16, N, k, 0xDEADBEEF))
-    assert np.array_equal(A, F.to_matrix(F.GhostPanels(C16, N, k, plan=make_plan(backend), seed=0xDEADBEEF, w=3)))
-    assert not np.array_equal(A, F.to_matrix(F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF + 1, w=w)))
+    assert np.array_equal(A, F.to_matrix(F.GhostPanels(F.device_randn, C16, N, k, plan=make_plan(backend), seed=0xDEADBEEF, w=3)))
+    assert not np.array_equal(A, F.to_matrix(F.GhostPanels(F.device_randn, C16, N, k, plan=plan, seed=0xDEADBEEF + 1, w=w)))
     assert 0.5 < np.linalg.norm(A) / np.sqrt(N * k) < 2
     assert np.linalg.matrix_rank(A) == min(N, k)
 
@@ -173,7 +173,7 @@ def test_ghost_feeds_sweeps_and_refuses_writes(backend):
     rng = np.random.default_rng(400)
     N, k, w = 40, 23, 7
     plan = make_plan(backend)
-    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    O = F.GhostPanels(F.device_randn, C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     A = OG.matrix(C16, N, k, 0xDEADBEEF)
     Gh = randmatrix(rng, N, N)
     G = F.DenseOperator(Gh, backend)
@@ -196,7 +196,7 @@ def test_sketch_chain_matches_the_oracle(backend):
     # The 1-D sketch chain at a small N: ghost -> tridiagonal -> cholqr2 -> project.
     N, k, w = 4096, 64, 8
     plan = make_plan(backend)
-    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    O = F.GhostPanels(F.device_randn, C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     G = F.Tridiagonal1D(N, 2 + 0.1j, -1.0, backend=backend)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
     F.panelmul(Y, G, O)

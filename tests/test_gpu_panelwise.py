@@ -116,7 +116,7 @@ def test_panelwise_matches_the_row_sweeps_on_a_ghost_basis(backend):
     N, k, w = 1 << 20, 128, 16
     print(f"panelwise ghost N={N} k={k}", flush=True)
     plan = F.ResidencyPlan(backend=backend, device_budget=32 << 30, host_budget=32 << 30)
-    O = F.GhostPanels(C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    O = F.GhostPanels(F.device_randn, C16, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     Y1 = F.PanelMatrix(N, k, plan=plan, w=w)
     Y2 = F.PanelMatrix(N, k, plan=plan, w=w)
     F.copyto(Y1, O)

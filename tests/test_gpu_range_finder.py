@@ -47,7 +47,7 @@ def test_range_finder_through_the_disk_tier(backend, tmp_path, nb):
     d = OO.helmholtz_diagonal()
     G = F.Helmholtz3D(n, n, n, d, -1.0, backend=backend)
     Gh = G.adjoint()
-    omega = F.GhostPanels(np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
+    omega = F.GhostPanels(F.device_randn, np.complex128, N, k, plan=plan, seed=0xDEADBEEF, w=w)
     Y = F.PanelMatrix(N, k, plan=plan, w=w)
     Z = F.PanelMatrix(N, k, plan=plan, w=w)
     try:
