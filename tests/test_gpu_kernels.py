@@ -330,11 +330,14 @@ def test_stencils(backend):
     assert F.check_operator(H, seed=1)
 
 
-@pytest.mark.parametrize("n,w", [((216, 216, 216), 2), ((37, 11, 5), 3), ((1, 1, 7), 2), ((5, 1, 1), 1)])
+@pytest.mark.parametrize("n,w", [((216, 216, 216), 2), ((37, 11, 5), 3), ((1, 1, 7), 2), ((5, 1, 1), 1),
+                                 ((2, 3, 50), 2), ((600, 3, 4), 2)])
 def test_helmholtz_rows_at_size(backend, n, w):
-    """The row-chunk stencil at the benchmark's 216^3 grid (each division by n1
-    and n2 runs over every row index below 10,077,696), on odd grids, and on
-    grids with an edge of 1, in views with leading dimensions above N."""
+    """The plane-blocked stencil at the benchmark's 216^3 grid (each division
+    by n1 runs over every point of a plane), on odd grids, on grids with an
+    edge of 1 or 2, over several groups of planes (n3 = 50), and, past
+    n1 = 512, the row-chunk stencil it falls back to; in views with leading
+    dimensions above N."""
     import funicular_hip as F
     rng = np.random.default_rng(20)
     N = n[0] * n[1] * n[2]
