@@ -443,7 +443,7 @@ class HIPBackend(DeviceBackend):
         hptr, hld = host_region(host)
         item = host.dtype.itemsize
         rows, cols = host.shape if host.ndim == 2 else (host.shape[0], 1)
-        stream = _stream_of(queue) if queue is not None else None
+        stream = _stream_of(queue) if queue is not None else current_stream()
         if host_is_src:
             call("fhip_memcpy2d_async", dev_ptr, dev_ld * item, hptr, hld * item, rows * item, cols, kind, stream)
         else:
@@ -451,8 +451,12 @@ class HIPBackend(DeviceBackend):
         return self._finish(queue)
 
     def _finish(self, queue):
+        # A copy without a queue is synchronous, on the calling thread's
+        # current stream: inside submit that is the queue's, so the copy is
+        # ordered after the work already queued there (the task-local stream
+        # of ext/FunicularCUDAExt.jl:83-89); outside, the null stream.
         if queue is None:
-            call("fhip_stream_sync", None)
+            call("fhip_stream_sync", current_stream())
             return None
         return self.record_event(queue)
 
@@ -460,7 +464,7 @@ class HIPBackend(DeviceBackend):
         """A bare copy between eltypes outside a sweep (ext/FunicularCUDAExt.jl:195-212):
         stage in the host eltype on the device, convert there, and wait."""
         staging = self.alloc_device(host.dtype, host.shape)
-        stream = _stream_of(queue) if queue is not None else None
+        stream = _stream_of(queue) if queue is not None else current_stream()
         if to_device:
             self._copy(staging.ptr, staging.ld, host, _lib.FHIP_H2D, None, True, staging)
             self.convert_copy(dev, staging, stream=stream)

@@ -199,6 +199,20 @@ class JitteryBackend(DeviceBackend):
             return None
         return q.put(fn)
 
+    def _copy(self, fn, queue=None):
+        # A copy with a queue is asynchronous on it; without one it is
+        # synchronous, on the current queue inside submit (after the work
+        # already queued there), as the HIP backend's copies are.
+        if queue is not None:
+            return queue.put(fn)
+        q = getattr(_tls, "queue", None)
+        if q is None:
+            fn()
+        else:
+            q.put(fn)
+            q.sync()
+        return None
+
     # -- copies --
     def h2d(self, dst, src, queue=None):
         src = np.asarray(src)
@@ -207,7 +221,7 @@ class JitteryBackend(DeviceBackend):
         if tuple(dst.shape) != tuple(src.shape):
             from funicular_hip.device_api import DimensionMismatch
             raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
-        return self._launch(lambda: np.copyto(_np(dst), src.astype(dst.dtype, copy=False)), queue)
+        return self._copy(lambda: np.copyto(_np(dst), src.astype(dst.dtype, copy=False)), queue)
 
     def d2h(self, dst, src, queue=None):
         if dst.ndim == 1:
@@ -215,13 +229,15 @@ class JitteryBackend(DeviceBackend):
         if tuple(dst.shape) != tuple(src.shape):
             from funicular_hip.device_api import DimensionMismatch
             raise DimensionMismatch(f"sizes differ: destination {dst.shape}, source {src.shape}")
-        return self._launch(lambda: np.copyto(dst, _np(src).astype(dst.dtype, copy=False)), queue)
+        return self._copy(lambda: np.copyto(dst, _np(src).astype(dst.dtype, copy=False)), queue)
 
     def d2d(self, dst, src, queue=None):
-        return self._launch(lambda: np.copyto(_np(dst), _np(src).astype(dst.dtype, copy=False)), queue)
+        return self._copy(lambda: np.copyto(_np(dst), _np(src).astype(dst.dtype, copy=False)), queue)
 
     def to_host(self, A):
-        return np.array(_np(A), order="F")
+        out = np.empty(A.shape, dtype=A.dtype, order="F")
+        self.d2h(out, A)
+        return out
 
     def todevice(self, A):
         A = np.asfortranarray(A)
