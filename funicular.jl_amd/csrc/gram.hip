@@ -37,19 +37,29 @@ struct GramShape;
 #ifndef FHIP_GRAM_SWIZZLE
 #define FHIP_GRAM_SWIZZLE 1
 #endif
+// Threads of a 64-tile workgroup: 256 (4 waves of 32 x 32 sub-tiles) or 512
+// (8 waves of 32 x 16 sub-tiles, 118 registers, four waves a SIMD; it
+// measured level with 256 in FP64 and slower in FP32, so 256 is the default).
+#ifndef FHIP_GRAM64_THREADS
+#define FHIP_GRAM64_THREADS 256
+#endif
 
 template <>
 struct GramShape<64> {
   static constexpr int KC = FHIP_GRAM64_KC;
   static constexpr int KSPLIT = 1;
-  static constexpr int MB = 2, NB = 2;
+  static constexpr int THREADS = FHIP_GRAM64_THREADS;
+  static constexpr int MB = 2, NB = THREADS == 256 ? 2 : 1;
+  static constexpr int WAVES_EU = THREADS == 256 ? 2 : 4;
   static constexpr int RESIDENT = FHIP_GRAM64_RESIDENT;
 };
 template <>
 struct GramShape<32> {
   static constexpr int KC = 32;
   static constexpr int KSPLIT = 4;
+  static constexpr int THREADS = 256;
   static constexpr int MB = 2, NB = 2;
+  static constexpr int WAVES_EU = 2;
   static constexpr int RESIDENT = 1024;
 };
 
@@ -63,7 +73,6 @@ constexpr int gram_kc() {
   return (sizeof(TC) == 4 && TILE == 64) ? FHIP_GRAM_KC_FP32 : GramShape<TILE>::KC;
 }
 
-constexpr int GRAM_THREADS = 256;
 constexpr int GRAM_MIN_CHUNK_ROWS = 256;
 
 // Upper-triangle tile t -> (ti, tj), tiles numbered column by column.
@@ -128,13 +137,14 @@ static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm, bool fp3
 }
 
 template <typename TIn, typename TC, bool CPLX, int TILE>
-__global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void gram_partial_kernel(
+__global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_waves_per_eu(GramShape<TILE>::WAVES_EU))) void gram_partial_kernel(
     const TIn *__restrict__ A, int64_t lda, const TIn *__restrict__ B, int64_t ldb, int64_t m, int ka,
     int kb, int64_t chunk_rows, int ntiles, int ntm, int herm, TC *__restrict__ partial) {
   constexpr int KC = gram_kc<TC, TILE>();
   constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
   constexpr int MB = GramShape<TILE>::MB;
   constexpr int NB = GramShape<TILE>::NB;
+  constexpr int GRAM_THREADS = GramShape<TILE>::THREADS;
   // Complex tiles use a swizzled LDS layout with unpadded rows: the 8-byte
   // swizzle for FP64 tiles of 32 or more columns, the 4-byte one for FP32
   // tiles of 64. The others pad each row by 4.
@@ -171,15 +181,18 @@ __global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(2)
   const int lane = t & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
-  // The wave's sub-tile (a quadrant, or the whole 32 tile) and the K steps it owns.
-  const int mi0 = (KSPLIT == 1) ? (TILE / 2) * (w & 1) : 0;
-  const int nj0 = (KSPLIT == 1) ? (TILE / 2) * (w >> 1) : 0;
+  // The wave's sub-tile (16 MB x 16 NB of the tile, or the whole 32 tile) and
+  // the K steps it owns.
+  constexpr int WROWS = TILE / (16 * MB);  // sub-tiles down a column of the tile
+  static_assert(KSPLIT > 1 || WROWS * (TILE / (16 * NB)) == GRAM_THREADS / 64, "the waves tile the tile");
+  const int mi0 = (KSPLIT == 1) ? 16 * MB * (w % WROWS) : 0;
+  const int nj0 = (KSPLIT == 1) ? 16 * NB * (w / WROWS) : 0;
   const int s_begin = (KSPLIT > 1) ? w : 0;
   const int s_stride = KSPLIT;
-  // In a diagonal tile of A^H A the quadrant strictly below the diagonal is
+  // In a diagonal tile of A^H A a sub-tile strictly below the diagonal is
   // never read (the reduce mirrors the upper triangle), so its wave only
   // stages slices.
-  const bool skip_quadrant = same && KSPLIT == 1 && mi0 > nj0;
+  const bool skip_quadrant = same && KSPLIT == 1 && mi0 >= nj0 + 16 * NB;
 
   typedef typename Mfma<TC>::acc acc_t;
   // Complex tiles keep the three 3M sums (p1, p2, p3); real tiles use p1 only.
@@ -382,7 +395,7 @@ static int launch_gram(const GramPlan &p, TOut *C, int64_t ldc, const TIn *A, in
                        double br, double bi, void *work, hipStream_t stream) {
   TC *partial = (TC *)work;
   const int grid = 8 * p.q * p.ntiles;
-  hipLaunchKernelGGL((gram_partial_kernel<TIn, TC, CPLX, TILE>), dim3(grid), dim3(GRAM_THREADS), 0,
+  hipLaunchKernelGGL((gram_partial_kernel<TIn, TC, CPLX, TILE>), dim3(grid), dim3(GramShape<TILE>::THREADS), 0,
                      stream, A, lda, B, ldb, m, ka, kb, p.chunk_rows, p.ntiles, p.ntm, herm ? 1 : 0,
                      partial);
   int st = check_launch("gram_partial_kernel");

@@ -41,13 +41,23 @@ namespace fhip {
 #define FHIP_TRSM_KC_FP32 16
 #endif
 constexpr int TW_THREADS = 256;  // 4 waves, each a 32 x 32 tile of the block
+constexpr int TW_CB = FHIP_TRSM_CB;
+constexpr int TW_ROWS = 4096 / TW_CB;
 // The solve Y R^-1 (trsm_wide_kernel) runs 8 waves, each a 32 x 16 tile of
 // the block: 128 registers a lane, so two workgroups (four waves a SIMD)
 // share a CU, against two waves a SIMD for 32 x 32 tiles at 234 registers.
-constexpr int TWW_THREADS = 512;
+// Its thread and row counts are tuning knobs: 1024 threads over 64 rows of
+// 128-column blocks, or over 128 rows of 64-column blocks, hold one
+// workgroup a CU and measured slower.
+#ifndef FHIP_TRSM_WTHREADS
+#define FHIP_TRSM_WTHREADS 512
+#endif
+#ifndef FHIP_TRSM_WROWS
+#define FHIP_TRSM_WROWS 64
+#endif
+constexpr int TWW_THREADS = FHIP_TRSM_WTHREADS;
+constexpr int TWW_ROWS = FHIP_TRSM_WROWS;  // rows per workgroup of trsm_wide_kernel
 constexpr int TWW_WAVES_EU = TWW_THREADS == 256 ? 2 : 4;
-constexpr int TW_CB = FHIP_TRSM_CB;
-constexpr int TW_ROWS = 4096 / TW_CB;
 #ifndef FHIP_TRSM_SWIZZLE
 #define FHIP_TRSM_SWIZZLE 1
 #endif
@@ -439,18 +449,19 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
     TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int TW_KC = TrsmSlice<T>::KC;
-  constexpr int TW_APT = TW_KC * TW_ROWS / TWW_THREADS;
+  constexpr int TW_APT = TW_KC * TWW_ROWS / TWW_THREADS;
   constexpr int TW_BPT = TW_KC * TW_CB / TWW_THREADS;
   constexpr int WNB = TWW_THREADS == 256 ? 2 : 1;  // 16-column tiles per wave
   constexpr int SWZ = !(CPLX && FHIP_TRSM_SWIZZLE) ? 0
                       : sizeof(T) == 8                                     ? 1
-                      : (TW_ROWS == 64 && TW_CB == 64 && TW_KC == 16)      ? 2
+                      : (TWW_ROWS == 64 && TW_CB == 64 && TW_KC == 16)      ? 2
                                                                            : 0;
-  constexpr int TW_LDA = SWZ ? TW_ROWS : TW_ROWS + 4;
+  constexpr int TW_LDA = SWZ ? TWW_ROWS : TWW_ROWS + 4;
   constexpr int TW_LDB = SWZ ? TW_CB : TW_CB + 4;
-  constexpr int YCS = TWW_THREADS / TW_ROWS;
+  constexpr int YCS = TWW_THREADS / TWW_ROWS;
   constexpr int MCS = TWW_THREADS / TW_KC;
-  static_assert(TWW_THREADS % TW_ROWS == 0 && TWW_THREADS % TW_KC == 0, "staging layout");
+  static_assert(TWW_THREADS % TWW_ROWS == 0 && TWW_THREADS % TW_KC == 0, "staging layout");
+  static_assert((TWW_THREADS / 64) / (TWW_ROWS / 32) * 16 * WNB == TW_CB, "the waves tile the block");
   static_assert(YCS * TW_APT == TW_KC && MCS * TW_BPT == TW_CB, "staging covers the slice");
   __shared__ T As[NP][TW_KC * TW_LDA];
   __shared__ T Bs[NP][TW_KC * TW_LDB];
@@ -458,18 +469,18 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
   typedef typename Pair<TS>::type PS;
   typedef typename Pair<T>::type PT;
 
-  const int64_t row0 = (int64_t)blockIdx.x * TW_ROWS;
+  const int64_t row0 = (int64_t)blockIdx.x * TWW_ROWS;
   const int t = threadIdx.x;
   const int w = t >> 6;
   const int lane = t & 63;
   const int li = lane & 15;
   const int lk = lane >> 4;
-  constexpr int WAVE_ROWS = TW_ROWS / 32;
+  constexpr int WAVE_ROWS = TWW_ROWS / 32;
   const int mi0 = 32 * (w % WAVE_ROWS);
   const int nj0 = 16 * WNB * (w / WAVE_ROWS);
   const int nbk = (k + TW_CB - 1) / TW_CB;
 
-  const int yr = t % TW_ROWS, yk = t / TW_ROWS;
+  const int yr = t % TWW_ROWS, yk = t / TWW_ROWS;
   const int mk = t % TW_KC, mc = t / TW_KC;
   const int64_t row = row0 + yr;
   const bool row_ok = row < m;
@@ -650,7 +661,7 @@ template <typename TS, typename T, bool CPLX, bool CT = false>
 static int launch_wide(TS *Y, int64_t ldy, int64_t m, int k, const T *M, hipStream_t stream) {
   if (m == 0) return FHIP_OK;
   const int64_t kp = trsm_kp(k);
-  const int64_t grid = cdiv64(m, TW_ROWS);
+  const int64_t grid = cdiv64(m, CT ? TW_ROWS : TWW_ROWS);
   if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
   if constexpr (CT)
     hipLaunchKernelGGL((trsm_wide_short_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream,
