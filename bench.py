@@ -830,6 +830,61 @@ def overlapped(backend, kernels, copies, fn):
                        "link_busy_fraction": t_c / wall if wall > 0 else None}
 
 
+def link_rates(nbytes=2 << 30, reps=3):
+    """The host link's copy rates with page-locked host memory, through the
+    library's own copy routine (fhip_memcpy2d_async, the copy engines):
+    nbytes up alone, nbytes down alone, and nbytes each way at once on two
+    streams. Best of `reps`, in GB/s."""
+    import ctypes
+    from funicular_hip import _lib
+    lib = _lib.load()
+    h_up, h_down, d_a, d_b = (ctypes.c_void_p() for _ in range(4))
+    _lib.call("fhip_host_alloc", ctypes.byref(h_up), nbytes)
+    _lib.call("fhip_host_alloc", ctypes.byref(h_down), nbytes)
+    _lib.call("fhip_malloc", ctypes.byref(d_a), nbytes)
+    _lib.call("fhip_malloc", ctypes.byref(d_b), nbytes)
+    s1, s2 = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.call("fhip_stream_create", ctypes.byref(s1))
+    _lib.call("fhip_stream_create", ctypes.byref(s2))
+
+    def up():
+        _lib.call("fhip_memcpy2d_async", d_a, nbytes, h_up, nbytes, nbytes, 1, _lib.FHIP_H2D, s1)
+
+    def down():
+        _lib.call("fhip_memcpy2d_async", h_down, nbytes, d_b, nbytes, nbytes, 1, _lib.FHIP_D2H, s2)
+
+    def best(*fns):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            for fn in fns:
+                fn()
+            _lib.call("fhip_stream_sync", s1)
+            _lib.call("fhip_stream_sync", s2)
+            ts.append(time.perf_counter() - t)
+        return min(ts)
+
+    up(); down()  # first touch
+    _lib.call("fhip_stream_sync", s1)
+    _lib.call("fhip_stream_sync", s2)
+    t_up, t_down, t_both = best(up), best(down), best(up, down)
+    for s in (s1, s2):
+        _lib.call("fhip_stream_destroy", s)
+    for d in (d_a, d_b):
+        _lib.call("fhip_free", d)
+    for h in (h_up, h_down):
+        _lib.call("fhip_host_free", h)
+    return {"bytes_each_way": nbytes, "h2d_alone_GBps": nbytes / t_up / 1e9, "d2h_alone_GBps": nbytes / t_down / 1e9,
+            "both_at_once_GBps": 2 * nbytes / t_both / 1e9}
+
+
+# Sweeps of one cholqr2 that only upload (Gram sweeps) and that upload and
+# download (solve sweeps), in basis sizes, per schedule: the reference's two
+# passes are a Gram sweep and a solve sweep each; the fused schedule folds the
+# second pass's Gram into the first solve sweep.
+CHOLQR2_SWEEPS = {"reference": (2, 2), "fused": (1, 2)}
+
+
 STREAMED_GRIDS = (200, 180)
 
 
@@ -851,6 +906,10 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
     from funicular_hip.executor import readpanels, rowtraversal, sweep
     backend = F.hip_backend()
     k, w = args.k, args.w
+    # The link's own rates, before the basis takes the page-lockable memory.
+    link = link_rates()
+    log(f"streamed: link alone {link['h2d_alone_GBps']:.1f} GB/s up, {link['d2h_alone_GBps']:.1f} GB/s down, "
+        f"{link['both_at_once_GBps']:.1f} GB/s both ways at once")
     tried = []
     plan = Y = None
     for n in grids:
@@ -909,7 +968,8 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
     out = {"grid": n, "N": N, "k": k, "panel_width": w, "basis_GB": N * k * 16 / 1e9,
            "budgets": "device 64 GiB, host 62 GiB page-locked (the box's limit is 64 GiB), FP64 host storage, "
                       "nbuffers 2",
-           "grids_refused": tried, "panelmul_s": t_panelmul, "d2h_GBps_panelmul": down_pm / t_panelmul / 1e9,
+           "grids_refused": tried, "link": link,
+           "panelmul_s": t_panelmul, "d2h_GBps_panelmul": down_pm / t_panelmul / 1e9,
            "copy_only_sweep_nb1_s": t_copy1, "copy_only_sweep_nb2_s": t_copy2,
            "h2d_GBps_copy_only_nb1": up1 / t_copy1 / 1e9, "h2d_GBps_copy_only_nb2": up2 / t_copy2 / 1e9,
            "gram_sweep": dict(gram_tl, wall_s=t_gram,
@@ -926,13 +986,24 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
         if phase_gap > 0:  # the check's Gram sweep as a phase of its own
             time.sleep(phase_gap)
         orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
+        # What the link alone needs for this schedule's bytes: upload-only
+        # sweeps at the one-way rate, upload-and-download sweeps at the
+        # both-ways rate (each basis byte up and one down).
+        basis = N * k * 16
+        n_up_only, n_duplex = CHOLQR2_SWEEPS[schedule]
+        ideal = n_up_only * basis / (link["h2d_alone_GBps"] * 1e9) + \
+            n_duplex * 2 * basis / (link["both_at_once_GBps"] * 1e9)
         out[f"cholqr2_{schedule}"] = dict(tl, s=t_chol, rows_per_s_pcie_inclusive=N / t_chol,
                                           h2d_GB=up / 1e9, d2h_GB=down / 1e9,
                                           h2d_GBps=up / t_chol / 1e9, d2h_GBps=down / t_chol / 1e9,
-                                          link_GBps=(up + down) / t_chol / 1e9, orthogonality=orth)
+                                          link_GBps=(up + down) / t_chol / 1e9, orthogonality=orth,
+                                          link_bound_s=ideal, fraction_of_link_bound=ideal / t_chol,
+                                          link_bound_definition=f"{n_up_only} upload-only sweeps at the one-way "
+                                          f"rate + {n_duplex} upload-and-download sweeps at the both-ways rate")
         log(f"streamed: cholqr2 ({schedule} schedule) {t_chol:.2f} s, {N / t_chol:.0f} rows/s, "
             f"up {up / 1e9:.1f} GB, down {down / 1e9:.1f} GB, link busy {tl['link_busy_fraction']:.3f}, "
-            f"kernels under copies {tl['hidden_compute']:.3f}, ||Q^H Q - I||_F {orth:.1e}")
+            f"kernels under copies {tl['hidden_compute']:.3f}, {ideal / t_chol:.2f} of the link-bound time "
+            f"{ideal:.2f} s, ||Q^H Q - I||_F {orth:.1e}")
     # project on the orthonormal basis: by runs of held panel images (fused),
     # and on request the reference's p (p + 1)-step schedule, which uploads the
     # basis p + 1 times.
