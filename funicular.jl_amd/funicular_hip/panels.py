@@ -658,12 +658,10 @@ def _collect_columns(pm, cols, out):
             continue
         panel = pm.panels[j]
         dst = out[:, lo - cols.start:hi - cols.start]
-        if isinstance(panel.tier, GhostTier) and panel.home.source.generator is not None:
-            from .ghosts import generate_panel
-            host = np.empty((pm.N, panel.width), dtype=host_storage_eltype(pm.plan, pm.dtype), order="F")
-            generate_panel(panel.home, host)
-            dst[...] = host[:, lo - pr.start:hi - pr.start]
-        elif isinstance(panel.tier, GhostTier):
+        # A ghost panel with a host generator is staged like a host panel:
+        # generated into a host block, where it stays until evicted, as
+        # Matrix(pm) leaves it in the reference (src/panels.jl:346-359).
+        if isinstance(panel.tier, GhostTier) and panel.home.source.generator is None:
             tmp = backend.alloc_device(pm.dtype, (pm.N, hi - lo))
             # The fill and the blocking copy back share the calling thread's
             # stream, so the copy sees the fill without a wait of its own.
