@@ -304,11 +304,14 @@ def write_panel(home, src):
 
 
 def disk_reads(pm):
-    """src/io.jl:134-144."""
+    """Panel reads from pm's disk store so far, 0 without a store
+    (src/io.jl:134-144)."""
     return 0 if pm.store is None else pm.store.reads
 
 
 def disk_writes(pm):
+    """Panel writes to pm's disk store so far, 0 without a store
+    (src/io.jl:134-144)."""
     return 0 if pm.store is None else pm.store.writes
 
 

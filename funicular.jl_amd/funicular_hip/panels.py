@@ -191,10 +191,13 @@ class PanelMatrix:
 
 
 def npanels(pm):
+    """The number of column panels of pm (src/panels.jl:131-137)."""
     return len(pm.panels)
 
 
 def panelwidth(pm, j=None):
+    """The panel width w of pm, or the width of panel j (the last one may be
+    narrower; src/panels.jl:147-158)."""
     if j is None:
         return pm.w
     checkpanel(pm, j)
