@@ -221,3 +221,52 @@ def test_host_full_without_spillable_panels(tmp_path):
             P.pin(plan, p)
     with pytest.raises(F.ArgumentError, match="no panel of that size is free to spill"):
         prefetch(pm, next(j for j, p in enumerate(pm.panels) if ondisk(p)))
+
+
+def test_disk_staging_under_jitter(tmp_path):
+    # test/test_disk.jl:110-127: a write sweep whose panels cycle through the
+    # scratch file, on jittered queues
+    from fakes import JitteryBackend
+    rng = np.random.default_rng(40)
+    N, k, w = 12, 20, 4
+    A = randmatrix(rng, N, k, C16)
+    for rep in range(10):
+        backend = JitteryBackend(jitter=2e-4, seed=rep)
+        d = tmp_path / f"rep{rep}"
+        d.mkdir()
+        plan = F.ResidencyPlan(backend=backend, device_budget=1 << 30, scratch_dir=str(d), panel_width=w,
+                               host_budget=panelbudget(4, N, w, C16))
+        pm = F.PanelMatrix.from_array(A, plan=plan, w=w)
+        F.foreachpanel(lambda j, block: backend.scale(block, 2.0), pm)
+        assert np.array_equal(to_matrix(pm), 2 * A)
+        assert all(p.pins == 0 for p in pm.panels)
+        free(pm)
+
+
+def test_concurrent_readers_of_one_store(tmp_path):
+    # test/test_disk.jl:238-259: four threads load one file read-only, each
+    # with room for three of its eight panels, and collect it five times
+    import threading
+    rng = np.random.default_rng(41)
+    N, k, w = 9, 24, 3
+    A = randmatrix(rng, N, k, C16)
+    path = str(tmp_path / "shared.fpan")
+    F.save(F.PanelMatrix.from_array(A, plan=F.ResidencyPlan(backend=HostOnlyBackend(), device_budget=1 << 30,
+                                                             host_budget=1 << 20), w=w), path)
+    results = [None] * 4
+
+    def reader(i):
+        plan = F.ResidencyPlan(backend=HostOnlyBackend(), device_budget=1 << 30, panel_width=w,
+                               host_budget=panelbudget(3, N, w, C16))
+        pm = F.load(path, plan=plan, readonly=True)
+        try:
+            results[i] = all(np.array_equal(to_matrix(pm), A) for _ in range(5))
+        finally:
+            free(pm)
+
+    threads = [threading.Thread(target=reader, args=(i,)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert results == [True] * 4
