@@ -331,13 +331,14 @@ def test_stencils(backend):
 
 
 @pytest.mark.parametrize("n,w", [((216, 216, 216), 2), ((37, 11, 5), 3), ((1, 1, 7), 2), ((5, 1, 1), 1),
-                                 ((2, 3, 50), 2), ((600, 3, 4), 2)])
+                                 ((2, 3, 50), 2), ((600, 3, 4), 2), ((16, 70, 3), 2), ((65, 9, 4), 2),
+                                 ((100, 45, 7), 1), ((17, 1, 30), 2), ((20, 2, 1), 2), ((15, 40, 3), 1)])
 def test_helmholtz_rows_at_size(backend, n, w):
-    """The plane-blocked stencil at the benchmark's 216^3 grid (each division
-    by n1 runs over every point of a plane), on odd grids, on grids with an
-    edge of 1 or 2, over several groups of planes (n3 = 50), and, past
-    n1 = 512, the row-chunk stencil it falls back to; in views with leading
-    dimensions above N."""
+    """The plane-blocked stencil at the benchmark's 216^3 grid, on odd grids,
+    on grids with an edge of 1 or 2, over several groups of planes (n3 = 30,
+    50); on 2-D plane tiles from n1 = 16 (a ragged last tile in i (65) and in
+    j (16 x 70, 100 x 45), one or two rows (n2 = 1, 2), one plane), and below
+    n1 = 16 on 1-D tiles; in views with leading dimensions above N."""
     import funicular_hip as F
     rng = np.random.default_rng(20)
     N = n[0] * n[1] * n[2]
