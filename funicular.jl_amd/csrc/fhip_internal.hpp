@@ -262,6 +262,29 @@ struct Pair<float> {
   typedef float type __attribute__((ext_vector_type(2)));
 };
 
+// Raw buffer loads. A descriptor covers nbytes from base; a load whose byte
+// offset lies past nbytes returns zero instead of touching memory, so a
+// descriptor ending where a slice's valid columns end zeroes the columns
+// past k without a per-lane test. The per-lane offset is a 32-bit VGPR and
+// the descriptor four SGPRs, so a slice's loads need no 64-bit address
+// arithmetic. Offsets are checked on the VGPR part alone: the SGPR offset
+// argument stays 0 and a slice moves its descriptor's base instead.
+// (0x00020000 is the gfx9 word 3 for raw, untyped access.)
+__device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)nbytes, 0x00020000);
+}
+
+template <typename P>
+__device__ inline P buffer_load_pair(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ inline Pair<double>::type buffer_load_pair<Pair<double>::type>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(Pair<double>::type, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+template <>
+__device__ inline Pair<float>::type buffer_load_pair<Pair<float>::type>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(Pair<float>::type, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
 // Loads one element of a real or complex matrix as (re, im) in type T. A
 // complex element is read as one aligned pair (ComplexF64 and ComplexF32 are
 // aligned to their full size).

@@ -444,9 +444,21 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 // of registers. Within a diagonal block, a 16-row slice of the triangular
 // M_JJ meets each 16-column tile of a wave as a zero block, a triangle or a
 // full block; the products with zero blocks are left out.
-template <typename TS, typename T, bool CPLX, bool CT>
+//
+// With BUF (complex storage, 16 columns of Y within a 32-bit byte range) the
+// slices are read with raw buffer loads: each thread's offsets into a slice
+// are fixed for the whole solve, a slice moves only the descriptors' bases
+// (scalar arithmetic), and the Y descriptor ends where the slice's valid
+// columns end, so columns past k read as zero without a per-lane select. Rows
+// past m are read unmasked (from the rows below, or as zeros past the end):
+// rows are independent and theirs are never stored. Every vector instruction
+// beside the matrix-core products costs matrix-core time on gfx950 (about
+// five cycles each), so the selects and 64-bit address updates this drops are
+// worth taking out of each slice.
+template <typename TS, typename T, bool CPLX, bool CT, bool BUF = false>
 __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW_WAVES_EU))) void trsm_wide_kernel(
     TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
+  static_assert(!BUF || CPLX, "buffer loads are for complex storage");
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int TW_KC = TrsmSlice<T>::KC;
   constexpr int TW_APT = TW_KC * TWW_ROWS / TWW_THREADS;
@@ -492,8 +504,28 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
   PS ya[2][TW_APT];
   PT ma[2][TW_BPT];
   bool oka[2][TW_APT];
+  // Buffer offsets of the thread's elements within a slice (BUF).
+  uint32_t yoff[TW_APT], moff[TW_BPT];
+#pragma unroll
+  for (int e = 0; e < TW_APT; ++e) yoff[e] = (uint32_t)((yr + (int64_t)(yk + YCS * e) * ldy) * (int64_t)sizeof(PS));
+#pragma unroll
+  for (int e = 0; e < TW_BPT; ++e) moff[e] = (uint32_t)((mk + (int64_t)(mc + MCS * e) * kp) * (int64_t)sizeof(PT));
   auto fetch = [&](auto set, int c0, int kc) {
     constexpr int S = decltype(set)::value;
+    if constexpr (BUF) {
+      // Valid columns of the slice, and the bytes from its base (row row0 of
+      // column kc) to the end of its last valid column.
+      const int nvalid = k - kc < TW_KC ? k - kc : TW_KC;
+      const int64_t ybytes = nvalid > 0 ? ((int64_t)nvalid * ldy - row0) * (int64_t)sizeof(PS) : 0;
+      const auto ry = buffer_rsrc(Y + (int64_t)NP * (row0 + (int64_t)kc * ldy), (uint32_t)ybytes);
+#pragma unroll
+      for (int e = 0; e < TW_APT; ++e) ya[S][e] = buffer_load_pair<PS>(ry, yoff[e]);
+      const int64_t mfirst = (int64_t)kc + (int64_t)c0 * kp;
+      const auto rm = buffer_rsrc(M + (int64_t)NP * mfirst, (uint32_t)((kp * kp - mfirst) * (int64_t)sizeof(PT)));
+#pragma unroll
+      for (int e = 0; e < TW_BPT; ++e) ma[S][e] = buffer_load_pair<PT>(rm, moff[e]);
+      return;
+    }
     const TS *yp = ybase + (int64_t)kc * ystride;
 #pragma unroll
     for (int e = 0; e < TW_APT; ++e) {
@@ -526,7 +558,12 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
       const int kk = yk + YCS * e;
       const int at = kk * TW_LDA + (yr ^ swizzle_of<SWZ>(kk));
       T re, im;
-      from_raw(ya[S][e], oka[S][e], re, im);
+      if constexpr (BUF) {
+        re = (T)ya[S][e].x;
+        im = (T)ya[S][e].y;
+      } else {
+        from_raw(ya[S][e], oka[S][e], re, im);
+      }
       As[0][at] = re;
       if (CPLX) As[NP - 1][at] = im;
     }
@@ -663,12 +700,28 @@ static int launch_wide(TS *Y, int64_t ldy, int64_t m, int k, const T *M, hipStre
   const int64_t kp = trsm_kp(k);
   const int64_t grid = cdiv64(m, CT ? TW_ROWS : TWW_ROWS);
   if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
-  if constexpr (CT)
+  // The buffer path needs a slice of Y (TrsmSlice<T>::KC columns) and the
+  // folded factor inside a 32-bit byte range.
+  const int64_t slice_bytes = (int64_t)TrsmSlice<T>::KC * ldy * 2 * (int64_t)sizeof(TS);
+#ifndef FHIP_TRSM_BUF
+#define FHIP_TRSM_BUF 1
+#endif
+  const bool buf = FHIP_TRSM_BUF && CPLX && slice_bytes < (int64_t(1) << 32) &&
+                   kp * kp * 2 * (int64_t)sizeof(T) < (int64_t(1) << 32);
+  if constexpr (CT) {
     hipLaunchKernelGGL((trsm_wide_short_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream,
                        Y, ldy, m, k, M, kp);
-  else
+  } else if constexpr (CPLX) {
+    if (buf)
+      hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT, true>), dim3((unsigned)grid), dim3(TWW_THREADS), 0,
+                         stream, Y, ldy, m, k, M, kp);
+    else
+      hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TWW_THREADS), 0, stream,
+                         Y, ldy, m, k, M, kp);
+  } else {
     hipLaunchKernelGGL((trsm_wide_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TWW_THREADS), 0, stream, Y,
                        ldy, m, k, M, kp);
+  }
   return check_launch("trsm_wide_kernel");
 }
 
