@@ -139,7 +139,7 @@ static GramPlan gram_plan(int64_t m, int64_t ka, int64_t kb, bool herm, bool fp3
 template <typename TIn, typename TC, bool CPLX, int TILE>
 __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_waves_per_eu(GramShape<TILE>::WAVES_EU))) void gram_partial_kernel(
     const TIn *__restrict__ A, int64_t lda, const TIn *__restrict__ B, int64_t ldb, int64_t m, int ka,
-    int kb, int64_t chunk_rows, int ntiles, int ntm, int herm, TC *__restrict__ partial) {
+    int kb, int64_t chunk_rows, int ntiles, int ntm, int herm, int buf, TC *__restrict__ partial) {
   constexpr int KC = gram_kc<TC, TILE>();
   constexpr int KSPLIT = GramShape<TILE>::KSPLIT;
   constexpr int MB = GramShape<TILE>::MB;
@@ -241,8 +241,70 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
     }
   };
   const bool b_staged = CONVERTED || !same;
+  // The fast fetch of a full slice of a tile inside both matrices: raw buffer
+  // loads at offsets fixed per thread, the descriptors moved per slice, no
+  // row or column test and no 64-bit address arithmetic (vector instructions
+  // beside the matrix-core products cost matrix-core time on gfx950, about
+  // five cycles each). `buf` is the host's word that a tile's 64 columns lie
+  // within a 32-bit byte range; the last, partial slice of a chunk and tiles
+  // at the matrices' edges take the guarded fetch above. Only the FP32
+  // products of FP64 input take it: the FP64 tiles ran 4% slower with it
+  // (9.65 against 9.26 ms for A^H A of 629,856 x 512), the FP32 ones 5%
+  // faster (5.20 against 5.47 ms).
+  constexpr bool FAST_KIND = CPLX && CONVERTED && KSPLIT == 1;
+  const bool fast = FAST_KIND && buf && i0 + TILE <= ka && j0 + TILE <= kb;
+  uint32_t aoff[PER_THREAD], boff[PER_THREAD];
+#pragma unroll
+  for (int e = 0; e < PER_THREAD; ++e) {
+    const int idx = t + GRAM_THREADS * e;
+    const int kk = idx % KC;
+    const int c = idx / KC;
+    aoff[e] = (uint32_t)((kk + (int64_t)c * lda) * (int64_t)sizeof(P2));
+    boff[e] = (uint32_t)((kk + (int64_t)c * ldb) * (int64_t)sizeof(P2));
+  }
+  auto fetch_fast = [&](int64_t rb) {
+    if constexpr (FAST_KIND) {
+      const auto rsa = buffer_rsrc(A + 2 * (rb + (int64_t)i0 * lda),
+                                   (uint32_t)(((TILE - 1) * lda + KC) * (int64_t)sizeof(P2)));
+#pragma unroll
+      for (int e = 0; e < PER_THREAD; ++e) {
+        const P2 v = buffer_load_pair<P2>(rsa, aoff[e]);
+        if constexpr (CONVERTED) {
+          ra[e] = v;
+          oka[e] = true;
+        } else {
+          pa_re[e] = v.x;
+          pa_im[e] = v.y;
+        }
+      }
+      if (b_staged) {
+        const auto rsb = buffer_rsrc(B + 2 * (rb + (int64_t)j0 * ldb),
+                                     (uint32_t)(((TILE - 1) * ldb + KC) * (int64_t)sizeof(P2)));
+#pragma unroll
+        for (int e = 0; e < PER_THREAD; ++e) {
+          const P2 v = buffer_load_pair<P2>(rsb, boff[e]);
+          if constexpr (CONVERTED) {
+            rbv[e] = v;
+            okb[e] = true;
+          } else {
+            pb_re[e] = v.x;
+            pb_im[e] = v.y;
+          }
+        }
+      }
+    }
+  };
+  auto fetch_any = [&](int64_t rb) {
+    if constexpr (FAST_KIND) {
+      if (fast && rb + KC <= r1) {
+        fetch_fast(rb);
+        return;
+      }
+    }
+    fetch(rb);
+  };
 
-  if (r0 < r1) fetch(r0);
+  if (r0 < r1) fetch_any(r0);
   for (int64_t rb = r0; rb < r1; rb += KC) {
     __syncthreads();
 #pragma unroll
@@ -269,7 +331,7 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
       }
     }
     __syncthreads();
-    if (rb + KC < r1) fetch(rb + KC);
+    if (rb + KC < r1) fetch_any(rb + KC);
     const TC *bre = b_staged ? Bs[0] : As[0];
     const TC *bim = b_staged ? Bs[NP - 1] : As[NP - 1];
     if (skip_quadrant) continue;
@@ -395,8 +457,14 @@ static int launch_gram(const GramPlan &p, TOut *C, int64_t ldc, const TIn *A, in
                        double br, double bi, void *work, hipStream_t stream) {
   TC *partial = (TC *)work;
   const int grid = 8 * p.q * p.ntiles;
+#ifndef FHIP_GRAM_BUF
+#define FHIP_GRAM_BUF 1
+#endif
+  // The fast fetch's descriptors span a tile's 64 columns of A and of B.
+  const int64_t span_bytes = (int64_t)TILE * (lda > ldb ? lda : ldb) * 2 * (int64_t)sizeof(TIn);
+  const int buf = (FHIP_GRAM_BUF && CPLX && span_bytes < (int64_t(1) << 32)) ? 1 : 0;
   hipLaunchKernelGGL((gram_partial_kernel<TIn, TC, CPLX, TILE>), dim3(grid), dim3(GramShape<TILE>::THREADS), 0,
-                     stream, A, lda, B, ldb, m, ka, kb, p.chunk_rows, p.ntiles, p.ntm, herm ? 1 : 0,
+                     stream, A, lda, B, ldb, m, ka, kb, p.chunk_rows, p.ntiles, p.ntm, herm ? 1 : 0, buf,
                      partial);
   int st = check_launch("gram_partial_kernel");
   if (st) return st;

@@ -104,6 +104,28 @@ def test_gram_mixed_precision(backend):
     assert err > 1e-12  # it really ran in FP32
 
 
+@pytest.mark.parametrize("m,ka,kb", [(3000, 128, 64), (2500, 130, 70), (700, 64, 200)])
+def test_gram_mixed_precision_cross_views(backend, m, ka, kb):
+    """FP32 products of A^H B for two distinct ComplexF64 blocks held as views
+    of taller matrices (leading dimensions 1037 and 1500 rows past m), with
+    alpha and beta: full 64-column tiles, whose row slices take the unguarded
+    fetch, beside ragged ones."""
+    rng = np.random.default_rng(m + ka + kb)
+    Abig = randmatrix(rng, m + 1037, ka + 3, np.complex128)
+    Bbig = randmatrix(rng, m + 1500, kb + 1, np.complex128)
+    C0 = randmatrix(rng, ka, kb, np.complex128)
+    Ad, Bd = backend.todevice(Abig), backend.todevice(Bbig)
+    A = Abig[5:5 + m, 2:2 + ka]
+    B = Bbig[9:9 + m, 1:1 + kb]
+    Cd = backend.todevice(C0)
+    alpha, beta = 0.5 - 1.5j, 2.0 + 0.25j
+    backend.gram_accumulate(Cd, Ad.view((5, 5 + m), (2, 2 + ka)), Bd.view((9, 9 + m), (1, 1 + kb)),
+                            alpha, beta, compute=np.complex64)
+    ref = alpha * (A.conj().T @ B) + beta * C0
+    err = np.linalg.norm(backend.to_host(Cd) - ref) / np.linalg.norm(ref)
+    assert 1e-12 < err < 1e-5
+
+
 @pytest.mark.parametrize("T", EL)
 def test_rightmul_gemm(backend, T):
     rng = np.random.default_rng(15)
