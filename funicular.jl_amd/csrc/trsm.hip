@@ -448,10 +448,13 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 // With BUF (complex storage, 16 columns of Y within a 32-bit byte range) the
 // slices are read with raw buffer loads: each thread's offsets into a slice
 // are fixed for the whole solve, a slice moves only the descriptors' bases
-// (scalar arithmetic), and the Y descriptor ends where the slice's valid
-// columns end, so columns past k read as zero without a per-lane select. Rows
-// past m are read unmasked (from the rows below, or as zeros past the end):
-// rows are independent and theirs are never stored. Every vector instruction
+// (scalar arithmetic), and the Y descriptor ends at row m - 1 of the slice's
+// last valid column, so columns past k read as zero without a per-lane
+// select. Rows past m are read unmasked: in the last valid column they read
+// as zero, in the others they read the rows below within the same matrix
+// (rows are independent and theirs are never stored). The range never reaches
+// past the block's last element, so a row block at the bottom of a taller
+// matrix reads nothing past that matrix. Every vector instruction
 // beside the matrix-core products costs matrix-core time on gfx950 (about
 // five cycles each), so the selects and 64-bit address updates this drops are
 // worth taking out of each slice.
@@ -514,9 +517,11 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
     constexpr int S = decltype(set)::value;
     if constexpr (BUF) {
       // Valid columns of the slice, and the bytes from its base (row row0 of
-      // column kc) to the end of its last valid column.
+      // column kc) to the end of the block's last row in its last valid
+      // column: nothing past the block's last element, which for a row block
+      // inside a taller matrix is not the end of its column's ldy rows.
       const int nvalid = k - kc < TW_KC ? k - kc : TW_KC;
-      const int64_t ybytes = nvalid > 0 ? ((int64_t)nvalid * ldy - row0) * (int64_t)sizeof(PS) : 0;
+      const int64_t ybytes = nvalid > 0 ? ((int64_t)(nvalid - 1) * ldy + m - row0) * (int64_t)sizeof(PS) : 0;
       const auto ry = buffer_rsrc(Y + (int64_t)NP * (row0 + (int64_t)kc * ldy), (uint32_t)ybytes);
 #pragma unroll
       for (int e = 0; e < TW_APT; ++e) ya[S][e] = buffer_load_pair<PS>(ry, yoff[e]);

@@ -256,6 +256,31 @@ def test_rdiv_upper(backend, T):
     assert np.array_equal(H[3:], Y[3:])
 
 
+@pytest.mark.parametrize("compute", [None, np.complex64])
+def test_rdiv_upper_bottom_row_block_of_a_taller_matrix(backend, compute):
+    """Row blocks of a 4096 x 128 ComplexF64 matrix (an allocation of exactly
+    8 MiB, so it ends on a 2 MiB page boundary), solved as views with
+    ld = 4096: the bottom block of 1000 rows ends its last workgroup 24 rows
+    past the matrix, and those rows must not be read past the allocation.
+    Each block's result matches the solve of a standalone copy of it, and the
+    rows outside the view are untouched."""
+    rng = np.random.default_rng(31)
+    N, k = 4096, 128
+    Y = randmatrix(rng, N, k, np.complex128)
+    R = np.asfortranarray(np.triu(randmatrix(rng, k, k, np.complex128)) + 8 * np.eye(k))
+    Rd = backend.todevice(R)
+    for r0, r1 in ((N - 1000, N), (0, 1000), (1000, 3001)):
+        Yd = backend.todevice(Y)
+        backend.rdiv_upper(Yd.view((r0, r1), None), Rd, compute=compute)
+        alone = backend.todevice(np.asfortranarray(Y[r0:r1]))
+        backend.rdiv_upper(alone, Rd, compute=compute)
+        H = backend.to_host(Yd)
+        assert np.array_equal(H[r0:r1], backend.to_host(alone)), (r0, r1)
+        assert np.array_equal(H[:r0], Y[:r0]) and np.array_equal(H[r1:], Y[r1:])
+        bar = 1e-5 if compute is not None else tol(np.complex128, r1 - r0, k)
+        assert np.linalg.norm(H[r0:r1] @ R - Y[r0:r1]) <= bar * np.linalg.norm(Y[r0:r1])
+
+
 @pytest.mark.parametrize("m,k", [(1000, 64), (5000, 512), (333, 45), (100, 1), (200, 96)])
 def test_rdiv_upper_shapes(backend, m, k):
     rng = np.random.default_rng(m + k)
