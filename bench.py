@@ -903,7 +903,7 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
     that many seconds between phases, so a trace can be cut at the gaps."""
     import gc
     import funicular_hip as F
-    from funicular_hip.executor import readpanels, rowtraversal, sweep
+    from funicular_hip.executor import readpanels, rowtraversal, sweep, updatepanels
     backend = F.hip_backend()
     k, w = args.k, args.w
     # The link's own rates, before the basis takes the page-lockable memory.
@@ -955,6 +955,9 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
     noop = lambda *a: None
     t_copy1, up1, _, _ = timed(lambda: sweep(noop, readpanels(Y), nbuffers=1, traversal=traversal))
     t_copy2, up2, _, _ = timed(lambda: sweep(noop, readpanels(Y), nbuffers=2, traversal=traversal))
+    # Each row block up and back down unchanged, no kernels: the link's rate
+    # both ways at once in the sweeps' own copy pattern.
+    t_duplex2, upd, downd, _ = timed(lambda: sweep(noop, updatepanels(Y), nbuffers=2, traversal=traversal))
     kernels = KernelTimer(backend)
     copies = CopyTimer()
     if phase_gap > 0:
@@ -963,7 +966,8 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
     t_compute = gram_tl["kernel_s"]
     formula = (t_copy2 + t_compute - t_gram) / min(t_copy2, t_compute)
     log(f"streamed: copy-only sweep nb=1 {t_copy1:.2f} s ({up1 / t_copy1 / 1e9:.1f} GB/s), nb=2 {t_copy2:.2f} s "
-        f"({up2 / t_copy2 / 1e9:.1f} GB/s); gram sweep {t_gram:.2f} s with {t_compute:.3f} s of kernels, "
+        f"({up2 / t_copy2 / 1e9:.1f} GB/s), up and down {t_duplex2:.2f} s "
+        f"({(upd + downd) / t_duplex2 / 1e9:.1f} GB/s); gram sweep {t_gram:.2f} s with {t_compute:.3f} s of kernels, "
         f"{gram_tl['hidden_compute']:.3f} of them under copies")
     out = {"grid": n, "N": N, "k": k, "panel_width": w, "basis_GB": N * k * 16 / 1e9,
            "budgets": "device 64 GiB, host 62 GiB page-locked (the box's limit is 64 GiB), FP64 host storage, "
@@ -972,6 +976,8 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
            "panelmul_s": t_panelmul, "d2h_GBps_panelmul": down_pm / t_panelmul / 1e9,
            "copy_only_sweep_nb1_s": t_copy1, "copy_only_sweep_nb2_s": t_copy2,
            "h2d_GBps_copy_only_nb1": up1 / t_copy1 / 1e9, "h2d_GBps_copy_only_nb2": up2 / t_copy2 / 1e9,
+           "duplex_copy_only_sweep_nb2_s": t_duplex2,
+           "duplex_GBps_copy_only_nb2": (upd + downd) / t_duplex2 / 1e9,
            "gram_sweep": dict(gram_tl, wall_s=t_gram,
                               overlap_formula=formula,
                               overlap_formula_definition="(t_copy + t_compute - t_wall) / min(t_copy, t_compute), "
@@ -986,20 +992,18 @@ def streamed(args, phase_gap=0.0, grids=STREAMED_GRIDS, project_reference=False)
         if phase_gap > 0:  # the check's Gram sweep as a phase of its own
             time.sleep(phase_gap)
         orth = float(np.linalg.norm(F.gram(Y) - np.eye(k)))
-        # What the link alone needs for this schedule's bytes: upload-only
-        # sweeps at the one-way rate, upload-and-download sweeps at the
-        # both-ways rate (each basis byte up and one down).
-        basis = N * k * 16
+        # What the link alone needs for this schedule's bytes: its
+        # upload-only sweeps and upload-and-download sweeps, each timed as
+        # the copy-only sweep of that kind (same traversal, nbuffers 2).
         n_up_only, n_duplex = CHOLQR2_SWEEPS[schedule]
-        ideal = n_up_only * basis / (link["h2d_alone_GBps"] * 1e9) + \
-            n_duplex * 2 * basis / (link["both_at_once_GBps"] * 1e9)
+        ideal = n_up_only * t_copy2 + n_duplex * t_duplex2
         out[f"cholqr2_{schedule}"] = dict(tl, s=t_chol, rows_per_s_pcie_inclusive=N / t_chol,
                                           h2d_GB=up / 1e9, d2h_GB=down / 1e9,
                                           h2d_GBps=up / t_chol / 1e9, d2h_GBps=down / t_chol / 1e9,
                                           link_GBps=(up + down) / t_chol / 1e9, orthogonality=orth,
                                           link_bound_s=ideal, fraction_of_link_bound=ideal / t_chol,
-                                          link_bound_definition=f"{n_up_only} upload-only sweeps at the one-way "
-                                          f"rate + {n_duplex} upload-and-download sweeps at the both-ways rate")
+                                          link_bound_definition=f"{n_up_only} copy-only upload sweeps + "
+                                          f"{n_duplex} copy-only upload-and-download sweeps, nbuffers 2")
         log(f"streamed: cholqr2 ({schedule} schedule) {t_chol:.2f} s, {N / t_chol:.0f} rows/s, "
             f"up {up / 1e9:.1f} GB, down {down / 1e9:.1f} GB, link busy {tl['link_busy_fraction']:.3f}, "
             f"kernels under copies {tl['hidden_compute']:.3f}, {ideal / t_chol:.2f} of the link-bound time "
@@ -1049,7 +1053,7 @@ def streamed_narrow(args):
     backend.sync_all()
     t_panelmul = time.perf_counter() - t
     # The link's own rate in this process: a copy-only row sweep (nbuffers 1).
-    from funicular_hip.executor import readpanels, rowtraversal, sweep
+    from funicular_hip.executor import readpanels, rowtraversal, sweep, updatepanels
     c0 = F._lib.counters()
     t = time.perf_counter()
     sweep(lambda *a: None, readpanels(Y), nbuffers=1, traversal=rowtraversal(Y))
