@@ -445,16 +445,15 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 // M_JJ meets each 16-column tile of a wave as a zero block, a triangle or a
 // full block; the products with zero blocks are left out.
 //
-// With BUF (complex storage, 16 columns of Y within a 32-bit byte range) the
-// slices are read with raw buffer loads: each thread's offsets into a slice
-// are fixed for the whole solve, a slice moves only the descriptors' bases
-// (scalar arithmetic), and the Y descriptor ends at row m - 1 of the slice's
-// last valid column, so columns past k read as zero without a per-lane
-// select. Rows past m are read unmasked: in the last valid column they read
-// as zero, in the others they read the rows below within the same matrix
-// (rows are independent and theirs are never stored). The range never reaches
-// past the block's last element, so a row block at the bottom of a taller
-// matrix reads nothing past that matrix. Every vector instruction
+// With BUF (complex storage) the slices are read with raw buffer loads. A
+// wave stages one column of the block per element (its 64 lanes are the
+// block's 64 rows), so each of its Y loads gets a descriptor of its own,
+// built with scalar arithmetic: based at the column's row row0 and ranging
+// over the block's rows of that column that lie below m, or over nothing for a
+// column past k. The lane's offset, its row times the element size, is fixed
+// for the whole solve; rows past m and columns past k read as zero with no
+// per-lane select or 64-bit address arithmetic, and nothing outside the
+// block's own rows is read, whatever its leading dimension. Every vector instruction
 // beside the matrix-core products costs matrix-core time on gfx950 (about
 // five cycles each), so the selects and 64-bit address updates this drops are
 // worth taking out of each slice.
@@ -462,6 +461,7 @@ template <typename TS, typename T, bool CPLX, bool CT, bool BUF = false>
 __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW_WAVES_EU))) void trsm_wide_kernel(
     TS *__restrict__ Y, int64_t ldy, int64_t m, int k, const T *__restrict__ M, int64_t kp) {
   static_assert(!BUF || CPLX, "buffer loads are for complex storage");
+  static_assert(!BUF || TWW_ROWS == 64, "buffer loads take one column of the block per wave");
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int TW_KC = TrsmSlice<T>::KC;
   constexpr int TW_APT = TW_KC * TWW_ROWS / TWW_THREADS;
@@ -491,8 +491,10 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
   const int li = lane & 15;
   const int lk = lane >> 4;
   constexpr int WAVE_ROWS = TWW_ROWS / 32;
-  const int mi0 = 32 * (w % WAVE_ROWS);
-  const int nj0 = 16 * WNB * (w / WAVE_ROWS);
+  // The wave's tile offsets as scalars: the per-slice test of whether its
+  // column tile takes part then runs on the scalar unit.
+  const int mi0 = __builtin_amdgcn_readfirstlane(32 * (w % WAVE_ROWS));
+  const int nj0 = __builtin_amdgcn_readfirstlane(16 * WNB * (w / WAVE_ROWS));
   const int nbk = (k + TW_CB - 1) / TW_CB;
 
   const int yr = t % TWW_ROWS, yk = t / TWW_ROWS;
@@ -507,24 +509,24 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
   PS ya[2][TW_APT];
   PT ma[2][TW_BPT];
   bool oka[2][TW_APT];
-  // Buffer offsets of the thread's elements within a slice (BUF).
-  uint32_t yoff[TW_APT], moff[TW_BPT];
-#pragma unroll
-  for (int e = 0; e < TW_APT; ++e) yoff[e] = (uint32_t)((yr + (int64_t)(yk + YCS * e) * ldy) * (int64_t)sizeof(PS));
+  // Buffer loads (BUF): the wave's column within a slice (wave-uniform),
+  // the lane's byte offset in a column, the bytes of a column's rows below m,
+  // and the offsets of the thread's factor elements.
+  const int wcol = __builtin_amdgcn_readfirstlane(yk);
+  const uint32_t yoff = (uint32_t)(yr * sizeof(PS));
+  const uint32_t col_bytes = (uint32_t)((m - row0 < TWW_ROWS ? m - row0 : TWW_ROWS) * (int64_t)sizeof(PS));
+  uint32_t moff[TW_BPT];
 #pragma unroll
   for (int e = 0; e < TW_BPT; ++e) moff[e] = (uint32_t)((mk + (int64_t)(mc + MCS * e) * kp) * (int64_t)sizeof(PT));
   auto fetch = [&](auto set, int c0, int kc) {
     constexpr int S = decltype(set)::value;
     if constexpr (BUF) {
-      // Valid columns of the slice, and the bytes from its base (row row0 of
-      // column kc) to the end of the block's last row in its last valid
-      // column: nothing past the block's last element, which for a row block
-      // inside a taller matrix is not the end of its column's ldy rows.
-      const int nvalid = k - kc < TW_KC ? k - kc : TW_KC;
-      const int64_t ybytes = nvalid > 0 ? ((int64_t)(nvalid - 1) * ldy + m - row0) * (int64_t)sizeof(PS) : 0;
-      const auto ry = buffer_rsrc(Y + (int64_t)NP * (row0 + (int64_t)kc * ldy), (uint32_t)ybytes);
 #pragma unroll
-      for (int e = 0; e < TW_APT; ++e) ya[S][e] = buffer_load_pair<PS>(ry, yoff[e]);
+      for (int e = 0; e < TW_APT; ++e) {
+        const int col = kc + wcol + YCS * e;
+        const auto ry = buffer_rsrc(Y + (int64_t)NP * (row0 + (int64_t)col * ldy), col < k ? col_bytes : 0u);
+        ya[S][e] = buffer_load_pair<PS>(ry, yoff);
+      }
       const int64_t mfirst = (int64_t)kc + (int64_t)c0 * kp;
       const auto rm = buffer_rsrc(M + (int64_t)NP * mfirst, (uint32_t)((kp * kp - mfirst) * (int64_t)sizeof(PT)));
 #pragma unroll
@@ -656,6 +658,9 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
           nb_mask = CT ? (rb >= cb0 ? 1u : 0u) : (rb <= cb0 ? 1u : 0u);
       }
     }
+    // With one 16-column tile a wave, a tile's product is either whole or
+    // left out: one product path.
+    if constexpr (WNB == 1) contributes = contributes && nb_mask == FULL;
     if (contributes && nb_mask == FULL) {
       if constexpr (CPLX)
         mma_steps_3m<T, 2, WNB, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
@@ -663,7 +668,7 @@ __global__ __launch_bounds__(TWW_THREADS) __attribute__((amdgpu_waves_per_eu(TWW
       else
         mma_steps<T, false, 2, WNB, PROD_ADD>(As[0], As[0], TW_LDA, mi0, Bs[0], Bs[0], TW_LDB, nj0, 0, TW_KC / 4,
                                             1, p1, p2);
-    } else if (contributes) {
+    } else if (WNB > 1 && contributes) {
       if constexpr (CPLX)
         mma_steps_3m<T, 2, WNB, PROD_ADD, SWZ>(As[0], As[NP - 1], TW_LDA, mi0, Bs[0], Bs[NP - 1], TW_LDB, nj0, 0,
                                              TW_KC / 4, 1, p1, p2, p3, nb_mask);
@@ -705,14 +710,12 @@ static int launch_wide(TS *Y, int64_t ldy, int64_t m, int k, const T *M, hipStre
   const int64_t kp = trsm_kp(k);
   const int64_t grid = cdiv64(m, CT ? TW_ROWS : TWW_ROWS);
   if (grid > 0x7fffffff) return fail(FHIP_E_ARGUMENT, "fhip_trsm: too many rows (%lld)", (long long)m);
-  // The buffer path needs a slice of Y (TrsmSlice<T>::KC columns) and the
-  // folded factor inside a 32-bit byte range.
-  const int64_t slice_bytes = (int64_t)TrsmSlice<T>::KC * ldy * 2 * (int64_t)sizeof(TS);
+  // The buffer path needs the folded factor inside a 32-bit byte range (Y's
+  // descriptors each span one column of the block's rows).
 #ifndef FHIP_TRSM_BUF
 #define FHIP_TRSM_BUF 1
 #endif
-  const bool buf = FHIP_TRSM_BUF && CPLX && slice_bytes < (int64_t(1) << 32) &&
-                   kp * kp * 2 * (int64_t)sizeof(T) < (int64_t(1) << 32);
+  const bool buf = FHIP_TRSM_BUF && CPLX && TWW_ROWS == 64 && kp * kp * 2 * (int64_t)sizeof(T) < (int64_t(1) << 32);
   if constexpr (CT) {
     hipLaunchKernelGGL((trsm_wide_short_kernel<TS, T, CPLX, CT>), dim3((unsigned)grid), dim3(TW_THREADS), 0, stream,
                        Y, ldy, m, k, M, kp);
