@@ -185,8 +185,10 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
   // the K steps it owns.
   constexpr int WROWS = TILE / (16 * MB);  // sub-tiles down a column of the tile
   static_assert(KSPLIT > 1 || WROWS * (TILE / (16 * NB)) == GRAM_THREADS / 64, "the waves tile the tile");
-  const int mi0 = (KSPLIT == 1) ? 16 * MB * (w % WROWS) : 0;
-  const int nj0 = (KSPLIT == 1) ? 16 * NB * (w / WROWS) : 0;
+  // Scalar (wave-uniform) tile offsets: the tests built from them run on the
+  // scalar unit.
+  const int mi0 = __builtin_amdgcn_readfirstlane((KSPLIT == 1) ? 16 * MB * (w % WROWS) : 0);
+  const int nj0 = __builtin_amdgcn_readfirstlane((KSPLIT == 1) ? 16 * NB * (w / WROWS) : 0);
   const int s_begin = (KSPLIT > 1) ? w : 0;
   const int s_stride = KSPLIT;
   // In a diagonal tile of A^H A a sub-tile strictly below the diagonal is
@@ -241,34 +243,35 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
     }
   };
   const bool b_staged = CONVERTED || !same;
-  // The fast fetch of a full slice of a tile inside both matrices: raw buffer
-  // loads at offsets fixed per thread, the descriptors moved per slice, no
-  // row or column test and no 64-bit address arithmetic (vector instructions
-  // beside the matrix-core products cost matrix-core time on gfx950, about
-  // five cycles each). `buf` is the host's word that a tile's 64 columns lie
-  // within a 32-bit byte range; the last, partial slice of a chunk and tiles
-  // at the matrices' edges take the guarded fetch above. Only the FP32
-  // products of FP64 input take it: the FP64 tiles ran 4% slower with it
-  // (9.65 against 9.26 ms for A^H A of 629,856 x 512), the FP32 ones 5%
-  // faster (5.20 against 5.47 ms).
-  constexpr bool FAST_KIND = CPLX && CONVERTED && KSPLIT == 1;
+  // The fast fetch of a full slice of a tile inside both matrices, by raw
+  // buffer loads: a wave's element e covers 64 / KC whole columns of the
+  // slice (lane l: row l % KC of column l / KC of its patch), so each gets a
+  // descriptor of its own based at the patch (scalar arithmetic), and the
+  // lane's byte offset within a patch is fixed for the whole launch. No row or
+  // column test and no 64-bit address arithmetic per element (vector
+  // instructions beside the matrix-core products cost matrix-core time on
+  // gfx950, about five cycles each). `buf` is the host's word that a patch
+  // ((64 / KC - 1) columns and KC rows) lies within a 32-bit byte range; the
+  // last, partial slice of a chunk and tiles at the matrices' edges take the
+  // guarded fetch above. The FP32 products of FP64 input take it; FP64 tiles
+  // take it when built with FHIP_GRAM_FAST64.
+#ifndef FHIP_GRAM_FAST64
+#define FHIP_GRAM_FAST64 0
+#endif
+  constexpr bool FAST_KIND = CPLX && KSPLIT == 1 && (CONVERTED || FHIP_GRAM_FAST64) && 64 % KC == 0;
+  constexpr int CPW = 64 / KC;  // columns of a wave's patch
   const bool fast = FAST_KIND && buf && i0 + TILE <= ka && j0 + TILE <= kb;
-  uint32_t aoff[PER_THREAD], boff[PER_THREAD];
-#pragma unroll
-  for (int e = 0; e < PER_THREAD; ++e) {
-    const int idx = t + GRAM_THREADS * e;
-    const int kk = idx % KC;
-    const int c = idx / KC;
-    aoff[e] = (uint32_t)((kk + (int64_t)c * lda) * (int64_t)sizeof(P2));
-    boff[e] = (uint32_t)((kk + (int64_t)c * ldb) * (int64_t)sizeof(P2));
-  }
+  const int wv = __builtin_amdgcn_readfirstlane(w);
+  const uint32_t loff_a = (uint32_t)(((lane % KC) + (int64_t)(lane / KC) * lda) * (int64_t)sizeof(P2));
+  const uint32_t loff_b = (uint32_t)(((lane % KC) + (int64_t)(lane / KC) * ldb) * (int64_t)sizeof(P2));
   auto fetch_fast = [&](int64_t rb) {
     if constexpr (FAST_KIND) {
-      const auto rsa = buffer_rsrc(A + 2 * (rb + (int64_t)i0 * lda),
-                                   (uint32_t)(((TILE - 1) * lda + KC) * (int64_t)sizeof(P2)));
 #pragma unroll
       for (int e = 0; e < PER_THREAD; ++e) {
-        const P2 v = buffer_load_pair<P2>(rsa, aoff[e]);
+        const int cb = (64 * wv + GRAM_THREADS * e) / KC;
+        const auto rsa = buffer_rsrc(A + 2 * (rb + (int64_t)(i0 + cb) * lda),
+                                     (uint32_t)(((CPW - 1) * lda + KC) * (int64_t)sizeof(P2)));
+        const P2 v = buffer_load_pair<P2>(rsa, loff_a);
         if constexpr (CONVERTED) {
           ra[e] = v;
           oka[e] = true;
@@ -278,11 +281,12 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
         }
       }
       if (b_staged) {
-        const auto rsb = buffer_rsrc(B + 2 * (rb + (int64_t)j0 * ldb),
-                                     (uint32_t)(((TILE - 1) * ldb + KC) * (int64_t)sizeof(P2)));
 #pragma unroll
         for (int e = 0; e < PER_THREAD; ++e) {
-          const P2 v = buffer_load_pair<P2>(rsb, boff[e]);
+          const int cb = (64 * wv + GRAM_THREADS * e) / KC;
+          const auto rsb = buffer_rsrc(B + 2 * (rb + (int64_t)(j0 + cb) * ldb),
+                                       (uint32_t)(((CPW - 1) * ldb + KC) * (int64_t)sizeof(P2)));
+          const P2 v = buffer_load_pair<P2>(rsb, loff_b);
           if constexpr (CONVERTED) {
             rbv[e] = v;
             okb[e] = true;
@@ -460,8 +464,11 @@ static int launch_gram(const GramPlan &p, TOut *C, int64_t ldc, const TIn *A, in
 #ifndef FHIP_GRAM_BUF
 #define FHIP_GRAM_BUF 1
 #endif
-  // The fast fetch's descriptors span a tile's 64 columns of A and of B.
-  const int64_t span_bytes = (int64_t)TILE * (lda > ldb ? lda : ldb) * 2 * (int64_t)sizeof(TIn);
+  // The fast fetch's descriptors span a wave's patch: 64 / KC columns of KC
+  // rows.
+  constexpr int KC = gram_kc<TC, TILE>();
+  const int64_t span_bytes =
+      ((int64_t)(64 / KC - 1) * (lda > ldb ? lda : ldb) + KC) * 2 * (int64_t)sizeof(TIn);
   const int buf = (FHIP_GRAM_BUF && CPLX && span_bytes < (int64_t(1) << 32)) ? 1 : 0;
   hipLaunchKernelGGL((gram_partial_kernel<TIn, TC, CPLX, TILE>), dim3(grid), dim3(GramShape<TILE>::THREADS), 0,
                      stream, A, lda, B, ldb, m, ka, kb, p.chunk_rows, p.ntiles, p.ntm, herm ? 1 : 0, buf,
