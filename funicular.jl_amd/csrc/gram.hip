@@ -253,12 +253,8 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
   // gfx950, about five cycles each). `buf` is the host's word that a patch
   // ((64 / KC - 1) columns and KC rows) lies within a 32-bit byte range; the
   // last, partial slice of a chunk and tiles at the matrices' edges take the
-  // guarded fetch above. The FP32 products of FP64 input take it; FP64 tiles
-  // take it when built with FHIP_GRAM_FAST64.
-#ifndef FHIP_GRAM_FAST64
-#define FHIP_GRAM_FAST64 0
-#endif
-  constexpr bool FAST_KIND = CPLX && KSPLIT == 1 && (CONVERTED || FHIP_GRAM_FAST64) && 64 % KC == 0;
+  // guarded fetch above.
+  constexpr bool FAST_KIND = CPLX && KSPLIT == 1 && 64 % KC == 0;
   constexpr int CPW = 64 / KC;  // columns of a wave's patch
   const bool fast = FAST_KIND && buf && i0 + TILE <= ka && j0 + TILE <= kb;
   const int wv = __builtin_amdgcn_readfirstlane(w);
@@ -298,19 +294,7 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
       }
     }
   };
-  auto fetch_any = [&](int64_t rb) {
-    if constexpr (FAST_KIND) {
-      if (fast && rb + KC <= r1) {
-        fetch_fast(rb);
-        return;
-      }
-    }
-    fetch(rb);
-  };
-
-  if (r0 < r1) fetch_any(r0);
-  for (int64_t rb = r0; rb < r1; rb += KC) {
-    __syncthreads();
+  auto stash = [&]() {
 #pragma unroll
     for (int e = 0; e < PER_THREAD; ++e) {
       const int idx = t + GRAM_THREADS * e;
@@ -334,17 +318,62 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
         }
       }
     }
-    __syncthreads();
-    if (rb + KC < r1) fetch_any(rb + KC);
+  };
+  auto multiply = [&]() {
     const TC *bre = b_staged ? Bs[0] : As[0];
     const TC *bim = b_staged ? Bs[NP - 1] : As[NP - 1];
-    if (skip_quadrant) continue;
+    if (skip_quadrant) return;
     if constexpr (CPLX)
       mma_steps_3m<TC, MB, NB, PROD_CONJ_A, SWZ>(As[0], As[NP - 1], LDS_W, mi0, bre, bim, LDS_W, nj0, s_begin,
                                                  KC / 4, s_stride, p1, p2, p3);
     else
       mma_steps<TC, false, MB, NB, PROD_CONJ_A>(As[0], As[0], LDS_W, mi0, bre, bre, LDS_W, nj0, s_begin,
                                                 KC / 4, s_stride, p1, p2);
+  };
+  // The slice loop, in two versions chosen once per workgroup so that each
+  // loop's staging registers have one kind of load behind them (loads of
+  // both kinds into the same registers make the compiler wait for them where
+  // the paths meet, before the products they were to overlap). The fast
+  // version runs the chunk's full slices and then its partial last slice, if
+  // any, with the guarded fetch.
+  auto run = [&](auto fastc) {
+    constexpr bool FAST = decltype(fastc)::value;
+    const int64_t rend = FAST ? r0 + (r1 - r0) / KC * KC : r1;
+    if (r0 < rend) {
+      if constexpr (FAST)
+        fetch_fast(r0);
+      else
+        fetch(r0);
+    }
+    for (int64_t rb = r0; rb < rend; rb += KC) {
+      __syncthreads();
+      stash();
+      __syncthreads();
+      if (rb + KC < rend) {
+        if constexpr (FAST)
+          fetch_fast(rb + KC);
+        else
+          fetch(rb + KC);
+      }
+      multiply();
+    }
+    if constexpr (FAST) {
+      if (rend < r1) {
+        fetch(rend);
+        __syncthreads();
+        stash();
+        __syncthreads();
+        multiply();
+      }
+    }
+  };
+  if constexpr (FAST_KIND) {
+    if (fast)
+      run(std::true_type());
+    else
+      run(std::false_type());
+  } else {
+    run(std::false_type());
   }
 
   // Partial slot of this workgroup (and wave, when waves split K).
