@@ -243,18 +243,23 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
     }
   };
   const bool b_staged = CONVERTED || !same;
-  // The fast fetch of a full slice of a tile inside both matrices, by raw
-  // buffer loads: a wave's element e covers 64 / KC whole columns of the
-  // slice (lane l: row l % KC of column l / KC of its patch), so each gets a
-  // descriptor of its own based at the patch (scalar arithmetic), and the
-  // lane's byte offset within a patch is fixed for the whole launch. No row or
-  // column test and no 64-bit address arithmetic per element (vector
-  // instructions beside the matrix-core products cost matrix-core time on
-  // gfx950, about five cycles each). `buf` is the host's word that a patch
-  // ((64 / KC - 1) columns and KC rows) lies within a 32-bit byte range; the
-  // last, partial slice of a chunk and tiles at the matrices' edges take the
-  // guarded fetch above.
+  // The fast fetch of a full slice of a tile inside both matrices: a wave's
+  // element e covers 64 / KC whole columns of the slice (lane l: row l % KC
+  // of column l / KC of its patch), so each is read from a base of its own at
+  // the patch (scalar arithmetic) plus the lane's byte offset within a patch,
+  // fixed for the whole launch. No row or column test and no 64-bit address
+  // arithmetic per element (vector instructions beside the matrix-core
+  // products cost matrix-core time on gfx950, about five cycles each). `buf`
+  // is the host's word that a patch ((64 / KC - 1) columns and KC rows) lies
+  // within a 32-bit byte range; the last, partial slice of a chunk and tiles
+  // at the matrices' edges take the guarded fetch above.
   constexpr bool FAST_KIND = CPLX && KSPLIT == 1 && 64 % KC == 0;
+  // The fast fetch reads only valid elements, so it needs no range check:
+  // FP64 tiles read each patch with global loads from its scalar base (8.75
+  // against 8.87 ms per 629,856-row block of the 216^3 basis, and 20 against
+  // 26 GB fetched from beyond L2); the FP32 products of FP64 data keep the
+  // buffer loads (4.97 against 5.11 ms).
+  constexpr bool FAST_GLOBAL = !CONVERTED;
   constexpr int CPW = 64 / KC;  // columns of a wave's patch
   const bool fast = FAST_KIND && buf && i0 + TILE <= ka && j0 + TILE <= kb;
   const int wv = __builtin_amdgcn_readfirstlane(w);
@@ -265,9 +270,12 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
 #pragma unroll
       for (int e = 0; e < PER_THREAD; ++e) {
         const int cb = (64 * wv + GRAM_THREADS * e) / KC;
-        const auto rsa = buffer_rsrc(A + 2 * (rb + (int64_t)(i0 + cb) * lda),
-                                     (uint32_t)(((CPW - 1) * lda + KC) * (int64_t)sizeof(P2)));
-        const P2 v = buffer_load_pair<P2>(rsa, loff_a);
+        const TIn *pa = A + 2 * (rb + (int64_t)(i0 + cb) * lda);
+        P2 v;
+        if constexpr (FAST_GLOBAL)
+          v = *reinterpret_cast<const P2 *>(reinterpret_cast<const char *>(pa) + loff_a);
+        else
+          v = buffer_load_pair<P2>(buffer_rsrc(pa, (uint32_t)(((CPW - 1) * lda + KC) * (int64_t)sizeof(P2))), loff_a);
         if constexpr (CONVERTED) {
           ra[e] = v;
           oka[e] = true;
@@ -280,9 +288,13 @@ __global__ __launch_bounds__(GramShape<TILE>::THREADS) __attribute__((amdgpu_wav
 #pragma unroll
         for (int e = 0; e < PER_THREAD; ++e) {
           const int cb = (64 * wv + GRAM_THREADS * e) / KC;
-          const auto rsb = buffer_rsrc(B + 2 * (rb + (int64_t)(j0 + cb) * ldb),
-                                       (uint32_t)(((CPW - 1) * ldb + KC) * (int64_t)sizeof(P2)));
-          const P2 v = buffer_load_pair<P2>(rsb, loff_b);
+          const TIn *pb = B + 2 * (rb + (int64_t)(j0 + cb) * ldb);
+          P2 v;
+          if constexpr (FAST_GLOBAL)
+            v = *reinterpret_cast<const P2 *>(reinterpret_cast<const char *>(pb) + loff_b);
+          else
+            v = buffer_load_pair<P2>(buffer_rsrc(pb, (uint32_t)(((CPW - 1) * ldb + KC) * (int64_t)sizeof(P2))),
+                                     loff_b);
           if constexpr (CONVERTED) {
             rbv[e] = v;
             okb[e] = true;
