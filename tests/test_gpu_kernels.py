@@ -104,12 +104,14 @@ def test_gram_mixed_precision(backend):
     assert err > 1e-12  # it really ran in FP32
 
 
+@pytest.mark.parametrize("compute", [None, np.complex64])
 @pytest.mark.parametrize("m,ka,kb", [(3000, 128, 64), (2500, 130, 70), (700, 64, 200)])
-def test_gram_mixed_precision_cross_views(backend, m, ka, kb):
-    """FP32 products of A^H B for two distinct ComplexF64 blocks held as views
-    of taller matrices (leading dimensions 1037 and 1500 rows past m), with
-    alpha and beta: full 64-column tiles, whose row slices take the unguarded
-    fetch, beside ragged ones."""
+def test_gram_cross_views(backend, m, ka, kb, compute):
+    """A^H B for two distinct ComplexF64 blocks held as views of taller
+    matrices (leading dimensions 1037 and 1500 rows past m), with alpha and
+    beta, in FP64 and in FP32 products: full 64-column tiles, whose full row
+    slices take the unguarded fetch, beside ragged tiles and partial last
+    slices, which take the guarded one."""
     rng = np.random.default_rng(m + ka + kb)
     Abig = randmatrix(rng, m + 1037, ka + 3, np.complex128)
     Bbig = randmatrix(rng, m + 1500, kb + 1, np.complex128)
@@ -120,10 +122,33 @@ def test_gram_mixed_precision_cross_views(backend, m, ka, kb):
     Cd = backend.todevice(C0)
     alpha, beta = 0.5 - 1.5j, 2.0 + 0.25j
     backend.gram_accumulate(Cd, Ad.view((5, 5 + m), (2, 2 + ka)), Bd.view((9, 9 + m), (1, 1 + kb)),
-                            alpha, beta, compute=np.complex64)
+                            alpha, beta, compute=compute)
     ref = alpha * (A.conj().T @ B) + beta * C0
     err = np.linalg.norm(backend.to_host(Cd) - ref) / np.linalg.norm(ref)
-    assert 1e-12 < err < 1e-5
+    if compute is None:
+        assert err < 1e-14
+    else:
+        assert 1e-12 < err < 1e-5
+
+
+@pytest.mark.parametrize("compute", [None, np.complex64])
+@pytest.mark.parametrize("m,k", [(3001, 130), (2048, 64), (5000, 200)])
+def test_gram_hermitian_row_block_view(backend, m, k, compute):
+    """A^H A of a row block held as a view of a taller, wider matrix (the
+    form every row sweep hands the Gram), in FP64 and FP32 products: the
+    diagonal tiles, which stage one operand, and the partial last slices of
+    the row chunks. The result is exactly Hermitian."""
+    rng = np.random.default_rng(m + k)
+    big = randmatrix(rng, m + 999, k + 2, np.complex128)
+    A = big[7:7 + m, 1:1 + k]
+    Ad = backend.todevice(big).view((7, 7 + m), (1, 1 + k))
+    C = backend.alloc_device(np.complex128, (k, k))
+    backend.gram_accumulate(C, Ad, Ad, 1.0, 0.0, compute=compute)
+    H = backend.to_host(C)
+    ref = A.conj().T @ A
+    err = np.linalg.norm(H - ref) / np.linalg.norm(ref)
+    assert np.array_equal(H, H.conj().T)
+    assert err < (1e-14 if compute is None else 1e-5)
 
 
 @pytest.mark.parametrize("T", EL)
