@@ -399,18 +399,22 @@ static uint32_t rows_grid(int64_t N, int64_t w, int64_t &chunks) {
 
 template <typename T, bool C>
 __global__ __launch_bounds__(256) void axpby_kernel(T *__restrict__ Y, int64_t ldy, const T *__restrict__ X,
-                                                    int64_t ldx, int64_t rows, int64_t cols, T ar, T ai, T br,
-                                                    T bi) {
-  const int64_t n = rows * cols;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
-    const int64_t i = e % rows;
-    const int64_t j = e / rows;
-    T xr, xi, yr, yi;
-    load_elem<T, T, C>(X, i + j * ldx, true, xr, xi);
-    load_elem<T, T, C>(Y, i + j * ldy, true, yr, yi);
-    const T outr = ar * xr - ai * xi + br * yr - bi * yi;
-    const T outi = ar * xi + ai * xr + br * yi + bi * yr;
-    store_elem<T, C>(Y, i + j * ldy, outr, outi);
+                                                    int64_t ldx, int64_t rows, int64_t nchunk, int64_t nids, T ar,
+                                                    T ai, T br, T bi) {
+  for (int64_t id = blockIdx.x; id < nids; id += gridDim.x) {
+    const int64_t j = id / nchunk;
+    const int64_t i0 = (id - j * nchunk) * ELT_ROWS + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < ELT_PER_THREAD; ++r) {
+      const int64_t i = i0 + 256 * r;
+      if (i >= rows) break;
+      T xr, xi, yr, yi;
+      load_elem<T, T, C>(X, i + j * ldx, true, xr, xi);
+      load_elem<T, T, C>(Y, i + j * ldy, true, yr, yi);
+      const T outr = ar * xr - ai * xi + br * yr - bi * yi;
+      const T outi = ar * xi + ai * xr + br * yi + bi * yr;
+      store_elem<T, C>(Y, i + j * ldy, outr, outi);
+    }
   }
 }
 
@@ -599,24 +603,25 @@ extern "C" int fhip_axpby(void *Y, int64_t ldy, const void *X, int64_t ldx, int6
   if (rows == 0 || cols == 0) return FHIP_OK;
   if (!Y || !X) return fail(FHIP_E_ARGUMENT, "fhip_axpby: null pointer");
   if (ldy < rows || ldx < rows) return fail(FHIP_E_ARGUMENT, "fhip_axpby: leading dimension smaller than the rows");
-  const unsigned g = grid_for(rows * cols);
+  const EltGrid g = elt_grid(rows, cols);
   hipStream_t s = (hipStream_t)stream;
   switch (type) {
     case FHIP_F32:
-      hipLaunchKernelGGL((axpby_kernel<float, false>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X,
-                         ldx, rows, cols, (float)a_re, 0.0f, (float)b_re, 0.0f);
+      hipLaunchKernelGGL((axpby_kernel<float, false>), dim3(g.blocks), dim3(256), 0, s, (float *)Y, ldy,
+                         (const float *)X, ldx, rows, g.nchunk, g.nids, (float)a_re, 0.0f, (float)b_re, 0.0f);
       break;
     case FHIP_F64:
-      hipLaunchKernelGGL((axpby_kernel<double, false>), dim3(g), dim3(256), 0, s, (double *)Y, ldy,
-                         (const double *)X, ldx, rows, cols, a_re, 0.0, b_re, 0.0);
+      hipLaunchKernelGGL((axpby_kernel<double, false>), dim3(g.blocks), dim3(256), 0, s, (double *)Y, ldy,
+                         (const double *)X, ldx, rows, g.nchunk, g.nids, a_re, 0.0, b_re, 0.0);
       break;
     case FHIP_C32:
-      hipLaunchKernelGGL((axpby_kernel<float, true>), dim3(g), dim3(256), 0, s, (float *)Y, ldy, (const float *)X,
-                         ldx, rows, cols, (float)a_re, (float)a_im, (float)b_re, (float)b_im);
+      hipLaunchKernelGGL((axpby_kernel<float, true>), dim3(g.blocks), dim3(256), 0, s, (float *)Y, ldy,
+                         (const float *)X, ldx, rows, g.nchunk, g.nids, (float)a_re, (float)a_im, (float)b_re,
+                         (float)b_im);
       break;
     case FHIP_C64:
-      hipLaunchKernelGGL((axpby_kernel<double, true>), dim3(g), dim3(256), 0, s, (double *)Y, ldy,
-                         (const double *)X, ldx, rows, cols, a_re, a_im, b_re, b_im);
+      hipLaunchKernelGGL((axpby_kernel<double, true>), dim3(g.blocks), dim3(256), 0, s, (double *)Y, ldy,
+                         (const double *)X, ldx, rows, g.nchunk, g.nids, a_re, a_im, b_re, b_im);
       break;
   }
   return check_launch("axpby_kernel");
